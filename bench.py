@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: batched FP32 forward+adjoint source-solves/s on the 3D diving-wave
+tomography workload (201 x 201 x 101, h = 25 m, 400 sources x 2000 receivers),
+against the single-source FP64 GPU form and the CPU path on the host cores.
+
+A step = one forward + adjoint + FP64 gradient/misfit reduction over all sources
+(every source of the shot set batched on the GPU), inputs resident in HBM.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "batched FP32 vs plain FSM fwd+adj source-solves/s (3D 201²×101); p99 lat"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, nthreads, nsamp, maxswp):
+    """The reference's own FSM/adjoint routines (oracle/_ref, compiled from the
+    reference sources) where present, else the FP64 oracle restatement, with the
+    reference's rank-per-contiguous-source-block layout over `nthreads` host
+    threads, on the first `nsamp` sources of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    vp = cfg["vp"]
+    nz, nr, nc = vp.shape
+    N = vp.size
+    h = cfg["h"]
+    src, srnum, rec, nxyz, tobs = cfg["src"], cfg["srnum"], cfg["rec"], cfg["nxyz"], cfg["T_obs"]
+    ref = None
+    if O.have_ref():
+        try:
+            ref = O.ref3d()
+        except OSError:
+            ref = None
+    O.oracle()
+    kind = "reference" if ref is not None else "port"
+    P = max(1, min(nthreads, nsamp))
+    per, rmd = divmod(nsamp, P)
+    blocks, s0 = [], 0
+    for r in range(P):
+        cnt = per + (1 if r < rmd else 0)
+        blocks.append((s0, s0 + cnt))
+        s0 += cnt
+    vpf = np.ascontiguousarray(vp.ravel())
+    srcf = np.ascontiguousarray(src.ravel())
+    recf = np.ascontiguousarray(rec.ravel())
+    nf = np.ascontiguousarray(nxyz.ravel())
+    grads = [np.zeros(N) for _ in range(P)]
+    mis = [0.0] * P
+
+    def rank(r):
+        b0, b1 = blocks[r]
+        T = np.empty(N)
+        lam = np.empty(N)
+        for s in range(b0, b1):
+            if ref is not None:
+                ref.ref_fsm3d(vpf, T, srcf, s, maxswp, h, h, h, nr, nc, nz)
+                mis[r] += ref.ref_adjoint_fsm3d(lam, T, tobs, recf, nf, srcf, srnum, s, maxswp,
+                                                h, h, h, nr, nc, nz)
+            else:
+                O.oracle().orc_fsm3d(vpf, T, srcf, s, maxswp, h, h, h, nr, nc, nz)
+                cyc = C.c_int(0)
+                mis[r] += O.oracle().orc_adjoint_fsm3d(lam, T, tobs, recf, nf, srcf, srnum, s,
+                                                       maxswp, h, h, h, nr, nc, nz, C.byref(cyc))
+            O.oracle().orc_grad_accum(grads[r], lam, vpf, N)
+
+    th = [threading.Thread(target=rank, args=(r,)) for r in range(P)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    return dict(value=nsamp / wall, unit="source-solves/s", cores=P, kind=kind,
+                sample=f"sources 0..{nsamp - 1} of the same 400-source workload, full FP64 "
+                       f"forward+adjoint+gradient each, {P} threads (rank per contiguous source "
+                       f"block), wall {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsrc", type=int, default=400)
+    ap.add_argument("--nrec", type=int, default=2000)
+    ap.add_argument("--maxswp", type=int, default=100)
+    ap.add_argument("--plain-sources", type=int, default=2,
+                    help="sources run through the single-source FP64 GPU form")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-sources", type=int, default=0, help="0 = one per thread")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the batched path keeps every source of a shot set "
+              "on a single MI355X and has no multi-GPU mode (see DESIGN.md for how it would "
+              "shard).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import pfast_amd
+    from pfast_amd import synth
+
+    t0 = time.perf_counter()
+    cfg = synth.config3d(nsrc=args.nsrc, nrec=args.nrec)
+    vp = cfg["vp"]
+    h = cfg["h"]
+    N = vp.size
+    log(f"workload generated in {time.perf_counter() - t0:.1f}s: grid {vp.shape}, "
+        f"{args.nsrc} sources x {args.nrec} receivers")
+    ctx = pfast_amd.Context(0)
+    model = pfast_amd.Model3D(ctx, vp, h, h, h)
+    shots = pfast_amd.Shots3D(model, cfg["src"], cfg["srnum"], cfg["rec"], cfg["nxyz"], cfg["T_obs"])
+    ctx.sync()
+    log("inputs resident in HBM")
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        shots.run(adjoint=True, MAXSwp=args.maxswp)
+        ctx.sync()
+        log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.3f}s")
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        shots.run(adjoint=True, MAXSwp=args.maxswp)
+        log(f"step {k + 1}/{args.steps} done")
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1000.0 * elapsed / args.steps
+    value = args.nsrc * args.steps / elapsed
+    cyc = shots.cycles()
+    grad, misfit = shots.gradient()
+    log(f"timed: {ms_step:.1f} ms/step, {value:.1f} source-solves/s; FP32 cycles fwd "
+        f"{cyc[:, 0].mean():.2f} adj {cyc[:, 1].mean():.2f}; misfit {misfit:.6g}")
+
+    # one more step with per-launch HIP events on the library's stream
+    ctx.timing(True)
+    shots.run(adjoint=True, MAXSwp=args.maxswp)
+    fwd_ms, fwd_n, fwd_units = ctx.timing_read(0)
+    adj_ms, adj_n, adj_units = ctx.timing_read(1)
+    ctx.timing(False)
+    B = args.nsrc
+    fwd_bytes = fwd_units * (8.0 + 4.0 / B)  # T read+write per source, slowness per model
+    adj_bytes = adj_units * 12.0             # lambda read+write, T read
+    fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
+    adj_gbs = adj_bytes / (adj_ms * 1e-3) / 1e9
+    dom = "adj3d_slab" if adj_ms >= fwd_ms else "fwd3d_slab"
+    log(f"kernels: fwd {fwd_ms:.1f} ms / {fwd_n} launches -> {fwd_gbs:.0f} GB/s; "
+        f"adj {adj_ms:.1f} ms / {adj_n} launches -> {adj_gbs:.0f} GB/s")
+
+    # single-source FP64 GPU form (bit-identical to the FP64 oracle) on a few sources
+    plain = []
+    terr, lerr = [], []
+    for s in range(args.plain_sources):
+        ctx.sync()
+        t = time.perf_counter()
+        EE, pc, T64, L64 = shots.plain_source(s, MAXSwp=args.maxswp, fields=True)
+        dt = time.perf_counter() - t
+        plain.append((dt, pc))
+        T32 = shots.field(s)
+        reach = T64 < 1e6
+        terr.append(float(np.abs(T32 - T64)[reach].max() / T64[reach].max()))
+        L32 = shots.field(s, adjoint=True)
+        lerr.append(float(np.linalg.norm(L32 - L64) / max(np.linalg.norm(L64), 1e-300)))
+        log(f"plain FP64 source {s}: {dt:.2f}s, cycles fwd {pc[0]} adj {pc[1]}; FP32 err T "
+            f"{terr[-1]:.2e} lambda {lerr[-1]:.2e}")
+    plain_rate = len(plain) / sum(p[0] for p in plain) if plain else None
+    If = float(np.mean([p[1][0] for p in plain])) if plain else None
+    Ia = float(np.mean([p[1][1] for p in plain])) if plain else None
+
+    # roofline of the dominant kernel (per-launch algorithmic bytes / mean launch time)
+    if dom == "adj3d_slab":
+        achieved, launches, avg_ms, per_launch = adj_gbs, adj_n, adj_ms / adj_n, adj_bytes / adj_n
+    else:
+        achieved, launches, avg_ms, per_launch = fwd_gbs, fwd_n, fwd_ms / fwd_n, fwd_bytes / fwd_n
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom, "launches": launches, "avg_launch_ms": round(avg_ms, 5),
+                "bytes_per_launch": round(per_launch)}
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "source-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded gradient+checkerboard model, analytic v(z) picks + 2 ms noise)",
+        "config": {"workload": "3D diving-wave tomography, 201x201x101 nodes, h=25 m, "
+                               f"{args.nsrc} sources x {args.nrec} receivers, forward+adjoint+gradient",
+                   "nsrc": args.nsrc, "nrec": args.nrec, "grid": [101, 201, 201], "h": h,
+                   "parallelism": "all sources batched on 1 GPU"},
+        "roofline": roofline,
+        "kernels": {"fwd3d_slab": {"ms": round(fwd_ms, 2), "launches": fwd_n,
+                                   "GBps": round(fwd_gbs, 1)},
+                    "adj3d_slab": {"ms": round(adj_ms, 2), "launches": adj_n,
+                                   "GBps": round(adj_gbs, 1)}},
+        "fp32_cycles_mean": [round(float(cyc[:, 0].mean()), 3), round(float(cyc[:, 1].mean()), 3)],
+        "misfit": misfit,
+    }
+    if plain:
+        bytes_solve = 8.0 * N * (If * (8.0 + 4.0 / B) + Ia * 12.0)
+        out["plain_fp64"] = {"value": round(plain_rate, 3), "unit": "source-solves/s",
+                             "sources": len(plain), "oracle_cycles": [If, Ia]}
+        out["ratio_vs_plain_fp64"] = round(value / plain_rate, 1)
+        out["roofline_solve"] = {"bytes_per_source_solve": bytes_solve,
+                                 "achieved_GBps": round(value * bytes_solve / 1e9, 1),
+                                 "frac": round(value * bytes_solve / 1e9 / HBM_PEAK_GBS, 4)}
+        out["accuracy_vs_fp64"] = {"T_maxnorm_rel": max(terr), "lambda_l2_rel": max(lerr)}
+    if not args.no_cpu:
+        ns = args.cpu_sources or args.cpu_threads
+        log(f"CPU baseline: {ns} sources on {args.cpu_threads} threads ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_threads, ns, args.maxswp)
+        log(f"CPU baseline: {out['cpu_baseline']['value']:.3f} source-solves/s")
+    shots.close()
+    model.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

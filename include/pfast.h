@@ -1,0 +1,159 @@
+/*
+ * pfast.h -- C ABI of libpfast, the MI355X (gfx950) traveltime-and-gradient engine.
+ *
+ * Plain C: pointers, sizes and int status codes.  Every routine returns
+ * PFAST_OK (0) or a negative code; pfast_last_error() describes the last failure
+ * on the calling thread.
+ *
+ * Two families of entry points:
+ *
+ *  1. Single-source FP64 forms, one source per call, host arrays in and out.
+ *     They replace the reference's subprograms one for one, with the same argument
+ *     meaning, layouts and results (bit-identical to the reference's FP64 code):
+ *       pfast_fsm3d            <- fsm3d            src/T_pfsm3d.h:131
+ *       pfast_adjoint_fsm3d    <- adjoint_fsm3d    src/T_padjoint3d.h:73
+ *       pfast_rt_fsm2d         <- RT_fsm2d         src/RT_pfsm2d.h:116
+ *       pfast_tadjoint_fsm2d   <- Tadjoint_fsm2d   src/RT_padjoint2d.h:288 (Cr = 1)
+ *                                                  and :67 (Cr = 1-WR)
+ *       pfast_radjoint_fsm2d   <- Radjoint_fsm2d   src/RT_padjoint2d.h:382 (Cr = 1)
+ *                                                  and :159 (Cr = WR)
+ *     The reference's fatal exits (`exit(1)`) become PFAST_ERR_FATAL.
+ *
+ *  2. Batched device-resident forms (FP32 sweeps, FP64 misfit/gradient reductions):
+ *     every source of a shot set on one GPU, model and fields resident in HBM.
+ *     They replace the per-source loops + MPI_Allreduce of the drivers
+ *     (src/T_PFAST3D.cpp:387-399, src/RT_PFAST2D.cpp:420-457).
+ *
+ * Layouts follow the reference: 3D node (i,j,k) = (row, col, depth) is stored at
+ * k*nrows*ncols + i*ncols + j; 2D node (i,j) = (row/depth, col) at i*ncols + j.
+ * src/rec index arrays are DIM-strided ints (3D: i,j,k; 2D: i,j).
+ */
+#ifndef PFAST_H
+#define PFAST_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFAST_OK 0
+#define PFAST_ERR_ARG (-1)   /* invalid argument */
+#define PFAST_ERR_HIP (-2)   /* HIP runtime failure / no device */
+#define PFAST_ERR_NOMEM (-3) /* device allocation failed */
+#define PFAST_ERR_FATAL (-4) /* a condition on which the reference calls exit(1) */
+
+#define PFAST_FWD_ONLY 1
+#define PFAST_FWD_ADJ 2
+
+const char *pfast_last_error(void);
+int pfast_version(void);
+/* Number of visible HIP devices (0 when none). */
+int pfast_device_count(int *n);
+
+/* ---------------------------------------------------------------------------------
+ * Single-source FP64 forms.  ncycles (may be NULL) receives the sweep cycles run.
+ * ------------------------------------------------------------------------------- */
+
+/* fsm3d, src/T_pfsm3d.h:131.  U (nz*nrows*ncols) is written. */
+int pfast_fsm3d(const double *vp, double *U, const int *src, int srcid, int MAXSwp,
+                double dx, double dy, double dz, int nrows, int ncols, int nz, int pe,
+                int *ncycles);
+
+/* adjoint_fsm3d, src/T_padjoint3d.h:73.  Writes lambda into U, misfit into *EE. */
+int pfast_adjoint_fsm3d(double *U, const double *T, const double *T_obs, const int *rec,
+                        const double *nxyz, const int *src, const int *srnum, int srcid,
+                        int MAXSwp, double dx, double dy, double dz, int nrows, int ncols,
+                        int nz, int pe, double *EE, int *ncycles);
+
+/* RT_fsm2d, src/RT_pfsm2d.h:116.  ncycles (may be NULL) has 1+2*NumofR entries:
+ * transmission, then (down, up) per reflector.  k2vec (may be NULL, ncols ints)
+ * receives the reflection row map of the last reflector.  rec/srnum are accepted
+ * for signature parity and unused, as in the reference. */
+int pfast_rt_fsm2d(const double *vp, double *U, double *TTd, double *TTu, const double *Z,
+                   const int *src, const int *rec, const int *srnum, int srcid, int NumofR,
+                   int MAXSwp, double dx, double dz, int nrows, int ncols, int pe,
+                   int *ncycles, int *k2vec);
+
+/* Tadjoint_fsm2d.  Cr = 1 for the plain overload (src/RT_padjoint2d.h:288),
+ * Cr = 1-WR for the WR overload (:67). */
+int pfast_tadjoint_fsm2d(double *U, const double *T, const double *T_obs, double Cr,
+                         const int *rec, const double *nxyz, const int *src, const int *srnum,
+                         int srcid, int MAXSwp, double dx, double dz, int nrows, int ncols,
+                         int pe, double *EE, int *ncycles);
+
+/* Radjoint_fsm2d.  Cr = 1 for the plain overload (src/RT_padjoint2d.h:382),
+ * Cr = WR for the WR overload (:159).  Td is accepted and unused, as in the
+ * reference (its Td adjoint is commented out).  ncycles has NumofR entries. */
+int pfast_radjoint_fsm2d(double *UR, const double *Td, const double *Tu, const double *TR_obs,
+                         double Cr, const int *rec, const double *nxyz, const int *src,
+                         const int *srnum, int srcid, int NumofR, int MAXSwp, double dx,
+                         double dz, int nrows, int ncols, int pe, double *EE, int *ncycles);
+
+/* ---------------------------------------------------------------------------------
+ * Batched, device-resident 3D path.
+ * ------------------------------------------------------------------------------- */
+typedef struct pfast_ctx pfast_ctx;
+typedef struct pfast_model3d pfast_model3d;
+typedef struct pfast_shots3d pfast_shots3d;
+
+int pfast_ctx_create(int device, pfast_ctx **out);
+int pfast_ctx_destroy(pfast_ctx *ctx);
+/* Blocks until all work queued by this context has finished. */
+int pfast_ctx_sync(pfast_ctx *ctx);
+
+/* A velocity model kept in HBM.  vp: nz*nrows*ncols doubles (reference layout). */
+int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols, int nz,
+                         double dx, double dy, double dz, pfast_model3d **out);
+/* Replace the velocity values (same grid), e.g. a line-search trial model. */
+int pfast_model3d_set_vp(pfast_model3d *m, const double *vp);
+int pfast_model3d_destroy(pfast_model3d *m);
+
+/* A shot set bound to a model: nsrc sources (src: 3*nsrc node indices, the
+ * reference's srcid order), srnum per-source receiver counts, and the flattened
+ * receiver table (rec: 3*nrec node indices, nxyz: 3*nrec normals (n_y,n_x,n_z),
+ * T_obs: nrec picks; a pick is used when T_obs < 100, as in the reference).
+ * T_obs/nxyz/rec may be NULL for forward-only use. */
+int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
+                         const int *rec, const double *nxyz, const double *T_obs,
+                         pfast_shots3d **out);
+int pfast_shots3d_destroy(pfast_shots3d *s);
+
+/* Run the batched FP32 forward (and, with PFAST_FWD_ADJ, the adjoint and the FP64
+ * gradient/misfit reductions) for every source of the set.  MAXSwp caps the sweep
+ * cycles per solve.  Results stay on the device until fetched. */
+int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp);
+
+/* Fetch results.  grad: N doubles (sum over sources of -lambda/v^3, the
+ * reference's global gradient before regularisation).  misfit: sum of EE.
+ * picks: sum(srnum) doubles, T at every receiver in receiver-table order.
+ * field: the FP32 traveltime (which=0) or adjoint (which=1) of source `src`
+ * widened to N doubles.  cycles: 2*nsrc ints (forward, adjoint cycles). */
+int pfast_shots3d_gradient(pfast_shots3d *s, double *grad, double *misfit);
+int pfast_shots3d_picks(pfast_shots3d *s, double *picks);
+int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field);
+int pfast_shots3d_cycles(pfast_shots3d *s, int *cycles);
+
+/* The single-source FP64 form on the same resident inputs: forward + adjoint of
+ * source `srcid` (the body of the reference's per-source loop,
+ * src/T_PFAST3D.cpp:388-391), one source per launch sequence, FP64 arithmetic
+ * identical to pfast_fsm3d/pfast_adjoint_fsm3d.  EE receives the misfit,
+ * cycles2 (may be NULL) the forward and adjoint cycle counts, T_out/lam_out
+ * (may be NULL, N doubles) the traveltime and adjoint fields. */
+int pfast_shots3d_plain_source(pfast_shots3d *s, int srcid, int MAXSwp, double *EE,
+                               int *cycles2, double *T_out, double *lam_out);
+
+/* Profiling: accumulated device time (ms) and launch count of the dominant sweep
+ * kernels of this context since the last reset, measured with HIP events on the
+ * context's stream.  kind 0 = forward sweeps, 1 = adjoint sweeps. */
+int pfast_ctx_timing(pfast_ctx *ctx, int kind, double *ms, long *launches);
+/* Node-sweeps (grid nodes x real sources x orderings) issued by the batched
+ * sweeps of `kind` since the last timing reset: the work the timed launches did. */
+int pfast_ctx_work(pfast_ctx *ctx, int kind, double *node_sweeps);
+int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable);
+int pfast_ctx_timing_reset(pfast_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,274 @@
+"""pfast_amd -- MI355X-native traveltime-and-gradient engine (FSM + adjoint-state).
+
+Python mirror of the reference's subprogram interface for this path, calling the
+HIP kernels of libpfast.so through its C ABI (include/pfast.h):
+
+    fsm3d            <- src/T_pfsm3d.h:131
+    adjoint_fsm3d    <- src/T_padjoint3d.h:73
+    RT_fsm2d         <- src/RT_pfsm2d.h:116
+    Tadjoint_fsm2d   <- src/RT_padjoint2d.h:288 (and the WR overload :67)
+    Radjoint_fsm2d   <- src/RT_padjoint2d.h:382 (and the WR overload :159)
+
+plus the batched device-resident shot-set API (Context / Model3D / Shots3D) that
+replaces the drivers' per-source loops and MPI reductions.
+
+Arrays use the reference layouts: 3D fields are numpy arrays of shape
+(nz, nrows, ncols), 2D fields (nrows, ncols); src/rec are int32 index tables
+(3D: row i, col j, depth k; 2D: row i, col j).  There is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (PFAST_FWD_ADJ, PFAST_FWD_ONLY, FatalError, PfastError, check, lib,  # noqa: F401
+                   ptr)
+
+INF = 9999999.0
+
+__all__ = ["fsm3d", "adjoint_fsm3d", "RT_fsm2d", "Tadjoint_fsm2d", "Radjoint_fsm2d",
+           "Context", "Model3D", "Shots3D", "device_count", "PfastError", "FatalError", "INF"]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().pfast_device_count(C.byref(n)))
+    return n.value
+
+
+# ----------------------------------------------------------------------- 3D
+def fsm3d(vp, src, srcid, MAXSwp, dx, dy, dz, pe=0):
+    """First-arrival traveltime of source `srcid` (FP64, bit-identical to the reference).
+    Returns (U, ncycles) with U shaped like vp."""
+    vp = _f64(vp)
+    nz, nrows, ncols = vp.shape
+    U = np.empty_like(vp)
+    cyc = C.c_int(0)
+    check(lib().pfast_fsm3d(vp.ravel(), U.ravel(), _i32(src).ravel(), int(srcid), int(MAXSwp),
+                            float(dx), float(dy), float(dz), nrows, ncols, nz, int(pe),
+                            C.byref(cyc)))
+    return U, cyc.value
+
+
+def adjoint_fsm3d(T, T_obs, rec, nxyz, src, srnum, srcid, MAXSwp, dx, dy, dz, pe=0):
+    """Adjoint state of source `srcid`.  Returns (lambda, EE, ncycles)."""
+    T = _f64(T)
+    nz, nrows, ncols = T.shape
+    U = np.empty_like(T)
+    T_obs, rec, nxyz = _f64(T_obs), _i32(rec), _f64(nxyz)
+    EE = C.c_double(0.0)
+    cyc = C.c_int(0)
+    check(lib().pfast_adjoint_fsm3d(U.ravel(), T.ravel(), ptr(T_obs), ptr(rec), ptr(nxyz),
+                                    _i32(src).ravel(), _i32(srnum), int(srcid), int(MAXSwp),
+                                    float(dx), float(dy), float(dz), nrows, ncols, nz, int(pe),
+                                    C.byref(EE), C.byref(cyc)))
+    return U, EE.value, cyc.value
+
+
+# ----------------------------------------------------------------------- 2D
+def RT_fsm2d(vp, Z, src, srcid, NumofR, MAXSwp, dx, dz, pe=0):
+    """Transmission + reflection traveltimes of source `srcid`.
+    Returns dict(T, TTd, TTu, cycles, k2vec).  Raises FatalError where the
+    reference exits."""
+    vp = _f64(vp)
+    nrows, ncols = vp.shape
+    R = max(int(NumofR), 1)
+    U = np.empty_like(vp)
+    TTd = np.zeros((R, nrows, ncols))
+    TTu = np.zeros((R, nrows, ncols))
+    cyc = np.zeros(1 + 2 * R, np.int32)
+    k2 = np.zeros(ncols, np.int32)
+    Zf = _f64(Z) if NumofR else np.zeros(3 * ncols)
+    check(lib().pfast_rt_fsm2d(vp.ravel(), U.ravel(), ptr(TTd), ptr(TTu), ptr(Zf),
+                               _i32(src).ravel(), None, None, int(srcid), int(NumofR),
+                               int(MAXSwp), float(dx), float(dz), nrows, ncols, int(pe),
+                               ptr(cyc), ptr(k2)))
+    return dict(T=U, TTd=TTd[:NumofR], TTu=TTu[:NumofR], cycles=cyc[:1 + 2 * NumofR], k2vec=k2)
+
+
+def Tadjoint_fsm2d(T, T_obs, rec, nxyz, src, srnum, srcid, MAXSwp, dx, dz, WR=None, pe=0):
+    """Transmission adjoint.  WR=None is the plain overload (Cr = 1), else the WR
+    overload (Cr = 1 - WR).  Returns (lambda, EE, ncycles)."""
+    T = _f64(T)
+    nrows, ncols = T.shape
+    U = np.empty_like(T)
+    Cr = 1.0 if WR is None else 1.0 - float(WR)
+    T_obs, rec, nxyz = _f64(T_obs), _i32(rec), _f64(nxyz)
+    EE = C.c_double(0.0)
+    cyc = np.zeros(1, np.int32)
+    check(lib().pfast_tadjoint_fsm2d(U.ravel(), T.ravel(), ptr(T_obs), Cr, ptr(rec), ptr(nxyz),
+                                     _i32(src).ravel(), _i32(srnum), int(srcid), int(MAXSwp),
+                                     float(dx), float(dz), nrows, ncols, int(pe), C.byref(EE),
+                                     ptr(cyc)))
+    return U, EE.value, int(cyc[0])
+
+
+def Radjoint_fsm2d(Tu, TR_obs, rec, nxyz, src, srnum, srcid, NumofR, MAXSwp, dx, dz, WR=None,
+                   pe=0):
+    """Reflection adjoint summed over reflectors.  WR=None: plain overload (Cr = 1),
+    else the WR overload (Cr = WR).  Returns (lambda_R, EE, cycles per reflector)."""
+    Tu = _f64(Tu)
+    nrows, ncols = Tu.shape[-2:]
+    UR = np.empty((nrows, ncols))
+    Cr = 1.0 if WR is None else float(WR)
+    TR_obs, rec, nxyz = _f64(TR_obs), _i32(rec), _f64(nxyz)
+    EE = C.c_double(0.0)
+    cyc = np.zeros(max(int(NumofR), 1), np.int32)
+    check(lib().pfast_radjoint_fsm2d(UR.ravel(), None, Tu.ravel(), ptr(TR_obs), Cr, ptr(rec),
+                                     ptr(nxyz), _i32(src).ravel(), _i32(srnum), int(srcid),
+                                     int(NumofR), int(MAXSwp), float(dx), float(dz), nrows, ncols,
+                                     int(pe), C.byref(EE), ptr(cyc)))
+    return UR, EE.value, cyc[:NumofR]
+
+
+# ------------------------------------------------------------------ batched
+class Context:
+    """A device stream plus profiling state (pfast_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().pfast_ctx_create(int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pfast_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().pfast_ctx_sync(self.h))
+
+    def timing(self, enable=True):
+        check(lib().pfast_ctx_timing_enable(self.h, 1 if enable else 0))
+        check(lib().pfast_ctx_timing_reset(self.h))
+
+    def timing_read(self, kind):
+        """(device ms, launches, node-sweeps) of sweep kind 0 (forward) / 1 (adjoint)."""
+        ms, n, w = C.c_double(0), C.c_long(0), C.c_double(0)
+        check(lib().pfast_ctx_timing(self.h, int(kind), C.byref(ms), C.byref(n)))
+        check(lib().pfast_ctx_work(self.h, int(kind), C.byref(w)))
+        return ms.value, n.value, w.value
+
+
+class Model3D:
+    """A velocity model resident in HBM (vp shaped (nz, nrows, ncols))."""
+
+    def __init__(self, ctx, vp, dx, dy, dz):
+        vp = _f64(vp)
+        self.ctx = ctx
+        self.shape = vp.shape
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        nz, nrows, ncols = vp.shape
+        h = C.c_void_p()
+        check(lib().pfast_model3d_create(ctx.h, vp.ravel(), nrows, ncols, nz, self.dx, self.dy,
+                                         self.dz, C.byref(h)))
+        self.h = h
+
+    def set_vp(self, vp):
+        vp = _f64(vp)
+        assert vp.shape == self.shape
+        check(lib().pfast_model3d_set_vp(self.h, vp.ravel()))
+
+    def close(self):
+        if self.h:
+            lib().pfast_model3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Shots3D:
+    """All sources of a shot set resident on one GPU.
+
+    src: (nsrc, 3) node indices (i, j, k); srnum: receivers per source; rec: (nrec, 3);
+    nxyz: (nrec, 3) normals (n_y, n_x, n_z); T_obs: (nrec,) picks (used where < 100).
+    """
+
+    def __init__(self, model, src, srnum, rec=None, nxyz=None, T_obs=None):
+        self.model = model
+        self.src = _i32(src).reshape(-1, 3)
+        self.nsrc = self.src.shape[0]
+        self.srnum = _i32(srnum)
+        self.nrec = int(self.srnum.sum())
+        self._rec = None if rec is None else _i32(rec).reshape(-1, 3)
+        self._nxyz = None if nxyz is None else _f64(nxyz).reshape(-1, 3)
+        self._tobs = None if T_obs is None else _f64(T_obs)
+        h = C.c_void_p()
+        check(lib().pfast_shots3d_create(model.h, self.nsrc, self.src.ravel(), self.srnum,
+                                         ptr(self._rec), ptr(self._nxyz), ptr(self._tobs),
+                                         C.byref(h)))
+        self.h = h
+
+    def run(self, adjoint=True, MAXSwp=100):
+        check(lib().pfast_shots3d_run(self.h, PFAST_FWD_ADJ if adjoint else PFAST_FWD_ONLY,
+                                      int(MAXSwp)))
+
+    def gradient(self):
+        N = int(np.prod(self.model.shape))
+        g = np.empty(N)
+        m = C.c_double(0)
+        check(lib().pfast_shots3d_gradient(self.h, ptr(g), C.byref(m)))
+        return g.reshape(self.model.shape), m.value
+
+    def misfit(self):
+        m = C.c_double(0)
+        check(lib().pfast_shots3d_gradient(self.h, None, C.byref(m)))
+        return m.value
+
+    def picks(self):
+        out = np.empty(self.nrec)
+        check(lib().pfast_shots3d_picks(self.h, out))
+        return out
+
+    def field(self, s, adjoint=False):
+        out = np.empty(int(np.prod(self.model.shape)))
+        check(lib().pfast_shots3d_field(self.h, 1 if adjoint else 0, int(s), out))
+        return out.reshape(self.model.shape)
+
+    def cycles(self):
+        c = np.zeros(2 * self.nsrc, np.int32)
+        check(lib().pfast_shots3d_cycles(self.h, c))
+        return c.reshape(-1, 2)
+
+    def plain_source(self, s, MAXSwp=100, fields=False):
+        """FP64 single-source forward+adjoint of source s on the resident inputs.
+        Returns (EE, cycles[2]) or, with fields=True, (EE, cycles, T, lambda)."""
+        EE = C.c_double(0)
+        cyc = np.zeros(2, np.int32)
+        T = lam = None
+        if fields:
+            T = np.empty(self.model.shape)
+            lam = np.empty(self.model.shape)
+        check(lib().pfast_shots3d_plain_source(self.h, int(s), int(MAXSwp), C.byref(EE),
+                                               ptr(cyc), ptr(T), ptr(lam)))
+        if fields:
+            return EE.value, cyc, T, lam
+        return EE.value, cyc
+
+    def close(self):
+        if self.h:
+            lib().pfast_shots3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

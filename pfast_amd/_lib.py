@@ -1,0 +1,104 @@
+"""ctypes binding of libpfast.so (the C ABI declared in include/pfast.h).
+
+The library is loaded from this package directory.  There is no fallback: if it
+is missing or fails to load, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfast.so")
+
+PFAST_OK = 0
+PFAST_ERR_ARG = -1
+PFAST_ERR_HIP = -2
+PFAST_ERR_NOMEM = -3
+PFAST_ERR_FATAL = -4
+PFAST_FWD_ONLY = 1
+PFAST_FWD_ADJ = 2
+
+
+class PfastError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpfast error {code}: {msg}")
+        self.code = code
+
+
+class FatalError(PfastError):
+    """A condition on which the reference program calls exit(1)."""
+
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+d, i, vpp = C.c_double, C.c_int, C.c_void_p
+pint, pdbl, plong = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_long)
+
+# name -> argtypes (all return int status)
+_SIGS = {
+    "pfast_version": [],
+    "pfast_device_count": [pint],
+    "pfast_fsm3d": [dp, dp, ip, i, i, d, d, d, i, i, i, i, vpp],
+    "pfast_adjoint_fsm3d": [dp, dp, vpp, vpp, vpp, ip, ip, i, i, d, d, d, i, i, i, i, pdbl, vpp],
+    "pfast_rt_fsm2d": [dp, dp, vpp, vpp, vpp, ip, vpp, vpp, i, i, i, d, d, i, i, i, vpp, vpp],
+    "pfast_tadjoint_fsm2d": [dp, dp, vpp, d, vpp, vpp, ip, ip, i, i, d, d, i, i, i, pdbl, vpp],
+    "pfast_radjoint_fsm2d": [dp, vpp, dp, vpp, d, vpp, vpp, ip, ip, i, i, i, d, d, i, i, i, pdbl, vpp],
+    "pfast_ctx_create": [i, C.POINTER(vpp)],
+    "pfast_ctx_destroy": [vpp],
+    "pfast_ctx_sync": [vpp],
+    "pfast_ctx_timing": [vpp, i, pdbl, plong],
+    "pfast_ctx_timing_enable": [vpp, i],
+    "pfast_ctx_timing_reset": [vpp],
+    "pfast_model3d_create": [vpp, dp, i, i, i, d, d, d, C.POINTER(vpp)],
+    "pfast_model3d_set_vp": [vpp, dp],
+    "pfast_model3d_destroy": [vpp],
+    "pfast_shots3d_create": [vpp, i, ip, ip, vpp, vpp, vpp, C.POINTER(vpp)],
+    "pfast_shots3d_destroy": [vpp],
+    "pfast_shots3d_run": [vpp, i, i],
+    "pfast_shots3d_gradient": [vpp, vpp, pdbl],
+    "pfast_shots3d_picks": [vpp, dp],
+    "pfast_shots3d_field": [vpp, i, i, dp],
+    "pfast_shots3d_cycles": [vpp, ip],
+    "pfast_shots3d_plain_source": [vpp, i, i, pdbl, vpp, vpp, vpp],
+    "pfast_ctx_work": [vpp, i, pdbl],
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libpfast (raises if the library is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PfastError(PFAST_ERR_HIP, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.pfast_last_error.argtypes = []
+        L.pfast_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS) + ["pfast_last_error"]
+
+
+def check(rc):
+    if rc != PFAST_OK:
+        msg = lib().pfast_last_error().decode(errors="replace")
+        if rc == PFAST_ERR_FATAL:
+            raise FatalError(rc, msg)
+        raise PfastError(rc, msg)
+    return rc
+
+
+def ptr(a):
+    """Address of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(vpp)

@@ -1,0 +1,40 @@
+// batch3d.h -- batched FP32 3D sweep launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+// Sources per chunk (innermost dimension of the batched field layout).
+#define PFAST_CHUNK 4
+
+struct SlabArgs {
+    float *field;         // [nchunk][N][C]: traveltimes (forward) or lambda (adjoint)
+    const float *tfield;  // adjoint only: traveltimes (negative marks receiver nodes)
+    const float *slow;    // [N] cell time h/v (uniform spacing) or slowness 1/v
+    const int *srcnode;   // [nchunk*C] linear source node, -1 for padding sources
+    const int *active;    // chunk ids still sweeping
+    unsigned *changed;    // [nchunk] bit c: source c of the chunk changed this cycle
+    int nact, nact_p8;    // active chunks; grid stride (multiple of 8 = XCD count)
+    int nrows, ncols, nz, nti, ntj;
+    float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
+};
+
+namespace pfast {
+hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D,
+                          int s1, int s2, int s3, int ntiles_diag);
+int tile_ti();
+int tile_tj();
+hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
+hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,
+                           int npick, long N, int unmark);
+hipError_t launch_adj_bc(hipStream_t st, float *Lm, const float *T, const long *bnode,
+                         const int *bsrc, const double *bobs, const double *bnrm, int nb,
+                         int nrows, int ncols, int nz, double dx, double dy, double dz);
+hipError_t launch_adj_misfit(hipStream_t st, const float *T, const long *pnode,
+                             const double *pobs, const long *poff, int nsrc, long N, double *EE);
+hipError_t launch_grad_reduce(hipStream_t st, const float *Lm, const double *vp, double *g,
+                              long N, int nchunk, int nsrc);
+hipError_t launch_gather_picks(hipStream_t st, const float *T, const long *rnode,
+                               const int *rsrc, int nrec, long N, double *out);
+hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, double *out);
+hipError_t launch_cell_time(hipStream_t st, const double *vp, float *f, long N, double h,
+                            int uniform);
+}  // namespace pfast

@@ -1,0 +1,972 @@
+// capi.hip -- extern "C" boundary of libpfast (see include/pfast.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pfast.h"
+#include "batch3d.h"
+#include "pfast_internal.h"
+#include "plain_kernels.h"
+
+namespace pfast {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace pfast
+
+// ------------------------------------------------------------------ context
+struct pfast_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    double ms[2] = {0, 0};
+    long launches[2] = {0, 0};
+    double units[2] = {0, 0};  // node-sweeps of real sources issued (batched path)
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, int>> pending;  // (kind, event index of start)
+    size_t evnext = 0;
+    // grow-only device scratch for the single-source forms
+    std::map<std::string, std::pair<void *, size_t>> cache;
+};
+
+namespace {
+
+int ensure_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        pfast::set_error("no HIP device visible");
+        return PFAST_ERR_HIP;
+    }
+    return PFAST_OK;
+}
+
+int scratch(pfast_ctx *ctx, const char *name, size_t bytes, void **out)
+{
+    auto &e = ctx->cache[name];
+    if (e.second < bytes) {
+        if (e.first) (void)hipFree(e.first);
+        e.first = nullptr;
+        e.second = 0;
+        if (hipMalloc(&e.first, bytes) != hipSuccess) {
+            pfast::set_error(std::string("hipMalloc failed for ") + name);
+            return PFAST_ERR_NOMEM;
+        }
+        e.second = bytes;
+    }
+    *out = e.first;
+    return PFAST_OK;
+}
+
+template <class T>
+int scratch_t(pfast_ctx *ctx, const char *name, size_t count, T **out)
+{
+    void *p = nullptr;
+    int rc = scratch(ctx, name, std::max<size_t>(count, 1) * sizeof(T), &p);
+    *out = (T *)p;
+    return rc;
+}
+
+std::mutex g_default_mu;
+pfast_ctx *g_default = nullptr;
+
+int default_ctx(pfast_ctx **out)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        int rc = pfast_ctx_create(dev, &g_default);
+        if (rc) return rc;
+    }
+    *out = g_default;
+    return PFAST_OK;
+}
+
+// timing helpers: an event pair around one launch when enabled
+void time_begin(pfast_ctx *ctx, int kind)
+{
+    if (!ctx->timing) return;
+    if (ctx->evnext + 2 > ctx->evpool.size()) {
+        size_t add = std::max<size_t>(256, ctx->evpool.size());
+        for (size_t i = 0; i < add; i++) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            ctx->evpool.push_back(e);
+        }
+    }
+    ctx->pending.push_back({kind, (int)ctx->evnext});
+    (void)hipEventRecord(ctx->evpool[ctx->evnext], ctx->stream);
+    ctx->evnext += 2;
+}
+
+void time_end(pfast_ctx *ctx)
+{
+    if (!ctx->timing) return;
+    (void)hipEventRecord(ctx->evpool[ctx->pending.back().second + 1], ctx->stream);
+}
+
+// After a stream sync: fold the recorded pairs into the totals.
+void time_collect(pfast_ctx *ctx)
+{
+    if (!ctx->timing) return;
+    for (auto &pr : ctx->pending) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->evpool[pr.second], ctx->evpool[pr.second + 1]);
+        ctx->ms[pr.first] += ms;
+        ctx->launches[pr.first] += 1;
+    }
+    ctx->pending.clear();
+    ctx->evnext = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pfast_last_error(void) { return pfast::g_err.c_str(); }
+int pfast_version(void) { return 10000; }
+
+int pfast_device_count(int *n)
+{
+    ARGCHK(n, "n is NULL");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return PFAST_OK;
+}
+
+int pfast_ctx_create(int device, pfast_ctx **out)
+{
+    ARGCHK(out, "out is NULL");
+    int rc = ensure_device();
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(device));
+    auto *c = new pfast_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        pfast::set_error("hipStreamCreate failed");
+        return PFAST_ERR_HIP;
+    }
+    *out = c;
+    return PFAST_OK;
+}
+
+int pfast_ctx_destroy(pfast_ctx *ctx)
+{
+    if (!ctx) return PFAST_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->cache)
+        if (e.second.first) (void)hipFree(e.second.first);
+    for (auto e : ctx->evpool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    if (ctx == g_default) g_default = nullptr;
+    delete ctx;
+    return PFAST_OK;
+}
+
+int pfast_ctx_sync(pfast_ctx *ctx)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    time_collect(ctx);
+    return PFAST_OK;
+}
+
+int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ctx->timing = enable != 0;
+    return PFAST_OK;
+}
+
+int pfast_ctx_timing_reset(pfast_ctx *ctx)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->pending.clear();
+    ctx->evnext = 0;
+    ctx->ms[0] = ctx->ms[1] = 0;
+    ctx->launches[0] = ctx->launches[1] = 0;
+    ctx->units[0] = ctx->units[1] = 0;
+    return PFAST_OK;
+}
+
+int pfast_ctx_work(pfast_ctx *ctx, int kind, double *node_sweeps)
+{
+    ARGCHK(ctx && kind >= 0 && kind < 2 && node_sweeps, "bad argument");
+    *node_sweeps = ctx->units[kind];
+    return PFAST_OK;
+}
+
+int pfast_ctx_timing(pfast_ctx *ctx, int kind, double *ms, long *launches)
+{
+    ARGCHK(ctx && kind >= 0 && kind < 2, "bad ctx/kind");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    time_collect(ctx);
+    if (ms) *ms = ctx->ms[kind];
+    if (launches) *launches = ctx->launches[kind];
+    return PFAST_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= single-source FP64 (3D)
+namespace {
+
+// Forward FSM of one source on device buffers (vp, U, p: N doubles; flag: 1 int).
+// Same cycle/convergence rule as src/T_pfsm3d.h:152.
+int fsm3d_dev(pfast_ctx *ctx, const double *vp, double *U, double *p, int *flag, int si, int sj,
+              int sk, int MAXSwp, double dx, double dy, double dz, int nrows, int ncols, int nz,
+              int *cycles_out)
+{
+    hipStream_t st = ctx->stream;
+    const long N = (long)nrows * ncols * nz;
+    HIPCHK(pfast::launch_init3d_f64(st, U, N, ((long)sk * nrows + si) * ncols + sj));
+    int count = 0;
+    for (;;) {
+        if (!(count < MAXSwp)) break;
+        count++;
+        HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), st));
+        for (int o = 0; o < 8; o++) {
+            const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+            time_begin(ctx, 0);
+            HIPCHK(pfast::launch_fsm3d_ordering_f64(st, vp, U, p, flag, si, sj, sk, s1, s2, s3, dx,
+                                                    dy, dz, nrows, ncols, nz, o == 0, o == 7));
+            time_end(ctx);
+        }
+        int h = 0;
+        HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        time_collect(ctx);
+        if (!h) break;
+    }
+    if (cycles_out) *cycles_out = count;
+    return PFAST_OK;
+}
+
+// Adjoint of one source on device buffers, src/T_padjoint3d.h:73-172.
+// T_obs/rec/nxyz point at this source's slice of the receiver table.
+int adj3d_dev(pfast_ctx *ctx, double *U, const double *T, unsigned char *tag, double *p,
+              int *flag, double *dEE, const double *T_obs, const int *rec, const double *nxyz,
+              int nrec, int si, int sj, int sk, int MAXSwp, double dx, double dy, double dz,
+              int nrows, int ncols, int nz, double *EE_out, int *cycles_out)
+{
+    hipStream_t st = ctx->stream;
+    const long N = (long)nrows * ncols * nz;
+    HIPCHK(hipMemsetAsync(U, 0, N * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(tag, 0, N, st));
+    HIPCHK(pfast::launch_adj3d_bc_f64(st, U, tag, T, T_obs, rec, nxyz, 0, nrec, si, sj, sk, dx,
+                                      dy, dz, nrows, ncols, nz, dEE, flag));
+    int notconv = 0;
+    double EE = 0;
+    HIPCHK(hipMemcpyAsync(&notconv, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&EE, dEE, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int count = 0, swept = 0;
+    for (;;) {
+        if (!(count < MAXSwp)) { count++; break; }
+        count++;
+        if (!notconv) break;
+        swept++;
+        HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), st));
+        for (int o = 0; o < 8; o++) {
+            const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+            time_begin(ctx, 1);
+            HIPCHK(pfast::launch_adj3d_ordering_f64(st, U, T, tag, p, flag, s1, s2, s3, dx, dy, dz,
+                                                    nrows, ncols, nz, o == 0, o == 7));
+            time_end(ctx);
+        }
+        HIPCHK(hipMemcpyAsync(&notconv, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        time_collect(ctx);
+    }
+    if (EE_out) *EE_out = EE;
+    if (cycles_out) *cycles_out = swept;
+    return PFAST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfast_fsm3d(const double *vp, double *U, const int *src, int srcid, int MAXSwp, double dx,
+                double dy, double dz, int nrows, int ncols, int nz, int pe, int *ncycles)
+{
+    (void)pe;
+    ARGCHK(vp && U && src, "NULL array");
+    ARGCHK(nrows >= 2 && ncols >= 2 && nz >= 2, "grid dimensions must be >= 2");
+    ARGCHK(srcid >= 0, "srcid < 0");
+    const int si = src[srcid * 3], sj = src[srcid * 3 + 1], sk = src[srcid * 3 + 2];
+    ARGCHK(si >= 0 && si < nrows && sj >= 0 && sj < ncols && sk >= 0 && sk < nz,
+           "source node outside the grid");
+    pfast_ctx *ctx;
+    int rc = default_ctx(&ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const long N = (long)nrows * ncols * nz;
+    double *dvp, *dU, *dp;
+    int *flag;
+    if ((rc = scratch_t(ctx, "p3.vp", N, &dvp)) || (rc = scratch_t(ctx, "p3.U", N, &dU)) ||
+        (rc = scratch_t(ctx, "p3.p", N, &dp)) || (rc = scratch_t(ctx, "p3.flag", 4, &flag)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(dvp, vp, N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = fsm3d_dev(ctx, dvp, dU, dp, flag, si, sj, sk, MAXSwp, dx, dy, dz, nrows, ncols, nz,
+                   ncycles);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(U, dU, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return PFAST_OK;
+}
+
+int pfast_adjoint_fsm3d(double *U, const double *T, const double *T_obs, const int *rec,
+                        const double *nxyz, const int *src, const int *srnum, int srcid,
+                        int MAXSwp, double dx, double dy, double dz, int nrows, int ncols, int nz,
+                        int pe, double *EE, int *ncycles)
+{
+    (void)pe;
+    ARGCHK(U && T && src && srnum, "NULL array");
+    ARGCHK(nrows >= 3 && ncols >= 3 && nz >= 3, "grid dimensions must be >= 3");
+    pfast_ctx *ctx;
+    int rc = default_ctx(&ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const long N = (long)nrows * ncols * nz;
+    long ch0 = 0;
+    for (int s = 0; s < srcid; s++) ch0 += srnum[s];
+    const int nrec = srnum[srcid];
+    ARGCHK(nrec == 0 || (T_obs && rec && nxyz), "NULL receiver table");
+    for (int r = 0; r < nrec; r++) {
+        const int *q = rec + (ch0 + r) * 3;
+        ARGCHK(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols && q[2] >= 0 && q[2] < nz,
+               "receiver node outside the grid");
+    }
+    double *dU, *dT, *dp, *dobs, *dn, *dEE;
+    int *drec, *flag;
+    unsigned char *tag;
+    if ((rc = scratch_t(ctx, "a3.U", N, &dU)) || (rc = scratch_t(ctx, "a3.T", N, &dT)) ||
+        (rc = scratch_t(ctx, "a3.p", N, &dp)) || (rc = scratch_t(ctx, "a3.tag", N, &tag)) ||
+        (rc = scratch_t(ctx, "a3.obs", nrec, &dobs)) ||
+        (rc = scratch_t(ctx, "a3.n", 3 * (size_t)nrec, &dn)) ||
+        (rc = scratch_t(ctx, "a3.rec", 3 * (size_t)nrec, &drec)) ||
+        (rc = scratch_t(ctx, "a3.EE", 1, &dEE)) || (rc = scratch_t(ctx, "a3.flag", 1, &flag)))
+        return rc;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dT, T, N * sizeof(double), hipMemcpyHostToDevice, st));
+    if (nrec) {
+        HIPCHK(hipMemcpyAsync(dobs, T_obs + ch0, nrec * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dn, nxyz + ch0 * 3, 3 * nrec * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(drec, rec + ch0 * 3, 3 * nrec * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    const int si = src[srcid * 3], sj = src[srcid * 3 + 1], sk = src[srcid * 3 + 2];
+    rc = adj3d_dev(ctx, dU, dT, tag, dp, flag, dEE, dobs, drec, dn, nrec, si, sj, sk, MAXSwp, dx,
+                   dy, dz, nrows, ncols, nz, EE, ncycles);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(U, dU, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+// ======================================================= single-source FP64 (2D)
+int pfast_rt_fsm2d(const double *vp, double *U, double *TTd, double *TTu, const double *Z,
+                   const int *src, const int *rec, const int *srnum, int srcid, int NumofR,
+                   int MAXSwp, double dx, double dz, int nrows, int ncols, int pe, int *ncycles,
+                   int *k2vec)
+{
+    (void)rec;
+    (void)srnum;
+    ARGCHK(vp && U && src, "NULL array");
+    ARGCHK(NumofR >= 0 && (NumofR == 0 || (TTd && TTu && Z)), "reflector arrays");
+    ARGCHK(nrows >= 2 && ncols >= 2, "grid dimensions must be >= 2");
+    const int si = src[srcid * 2], sj = src[srcid * 2 + 1];
+    ARGCHK(si >= 0 && si < nrows && sj >= 0 && sj < ncols, "source node outside the grid");
+    pfast_ctx *ctx;
+    int rc = default_ctx(&ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const long N = (long)nrows * ncols;
+    const int R = std::max(NumofR, 1);
+    double *dvp, *dU, *dTd, *dTu, *dp, *dZ;
+    int *dsrc, *dcyc, *dst, *dk2;
+    if ((rc = scratch_t(ctx, "r2.vp", N, &dvp)) || (rc = scratch_t(ctx, "r2.U", N, &dU)) ||
+        (rc = scratch_t(ctx, "r2.Td", N * R, &dTd)) || (rc = scratch_t(ctx, "r2.Tu", N * R, &dTu)) ||
+        (rc = scratch_t(ctx, "r2.p", N, &dp)) || (rc = scratch_t(ctx, "r2.Z", (size_t)R * ncols * 3, &dZ)) ||
+        (rc = scratch_t(ctx, "r2.src", 2, &dsrc)) || (rc = scratch_t(ctx, "r2.cyc", 1 + 2 * R, &dcyc)) ||
+        (rc = scratch_t(ctx, "r2.st", 1, &dst)) || (rc = scratch_t(ctx, "r2.k2", ncols, &dk2)))
+        return rc;
+    int hs[2] = {si, sj};
+    HIPCHK(hipMemcpyAsync(dvp, vp, N * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dsrc, hs, sizeof(hs), hipMemcpyHostToDevice, st));
+    if (NumofR)
+        HIPCHK(hipMemcpyAsync(dZ, Z, (size_t)NumofR * ncols * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    Grid2 g{nrows, ncols, dx, dz, 1.0 / (dx * dx), 1.0 / (dz * dz)};
+    HIPCHK(pfast::launch_rt_fsm2d_f64(st, dvp, dU, dTd, dTu, dp, dZ, dsrc, 1, NumofR, MAXSwp, g,
+                                      dcyc, dst, dk2));
+    int status = 0;
+    std::vector<int> cyc(1 + 2 * R);
+    HIPCHK(hipMemcpyAsync(&status, dst, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cyc.data(), dcyc, cyc.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(U, dU, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (NumofR) {
+        HIPCHK(hipMemcpyAsync(TTd, dTd, N * NumofR * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(TTu, dTu, N * NumofR * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (k2vec) HIPCHK(hipMemcpyAsync(k2vec, dk2, ncols * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ncycles) std::copy(cyc.begin(), cyc.begin() + 1 + 2 * NumofR, ncycles);
+    if (status == 1) {
+        pfast::set_error("no k2 above reflector inside model (src/RT_pfsm2d.h:232)");
+        return PFAST_ERR_FATAL;
+    }
+    if (status == 2) {
+        pfast::set_error("negative reflected traveltime (src/RT_pfsm2d.h:225)");
+        return PFAST_ERR_FATAL;
+    }
+    return PFAST_OK;
+}
+
+static int adj2d_common(bool refl, double *Uout, const double *T, const double *obs, double Cr,
+                        const int *rec, const double *nxyz, const int *src, const int *srnum,
+                        int srcid, int NumofR, int MAXSwp, double dx, double dz, int nrows,
+                        int ncols, double *EE, int *ncycles)
+{
+    ARGCHK(Uout && T && src && srnum, "NULL array");
+    ARGCHK(nrows >= 3 && ncols >= 3, "grid dimensions must be >= 3");
+    pfast_ctx *ctx;
+    int rc = default_ctx(&ctx);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const long N = (long)nrows * ncols;
+    long ch0 = 0;
+    for (int s = 0; s < srcid; s++) ch0 += srnum[s];
+    const int nrec = srnum[srcid];
+    const int R = refl ? std::max(NumofR, 1) : 1;
+    const int ostride = refl ? NumofR : 1;
+    ARGCHK(nrec == 0 || (obs && rec && nxyz), "NULL receiver table");
+    for (int r = 0; r < nrec; r++) {
+        const int *q = rec + (ch0 + r) * 2;
+        ARGCHK(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols, "receiver node outside the grid");
+    }
+    double *dU, *dU2, *dT, *dp, *dobs, *dn, *dEE;
+    int *drec, *dsrc, *dsr, *dcyc;
+    long *dch0;
+    unsigned char *tag;
+    if ((rc = scratch_t(ctx, "j2.U", N, &dU)) || (rc = scratch_t(ctx, "j2.U2", N, &dU2)) ||
+        (rc = scratch_t(ctx, "j2.T", N * R, &dT)) || (rc = scratch_t(ctx, "j2.p", N, &dp)) ||
+        (rc = scratch_t(ctx, "j2.tag", N, &tag)) ||
+        (rc = scratch_t(ctx, "j2.obs", (size_t)nrec * ostride, &dobs)) ||
+        (rc = scratch_t(ctx, "j2.n", 2 * (size_t)nrec, &dn)) ||
+        (rc = scratch_t(ctx, "j2.rec", 2 * (size_t)nrec, &drec)) ||
+        (rc = scratch_t(ctx, "j2.src", 2, &dsrc)) || (rc = scratch_t(ctx, "j2.sr", 1, &dsr)) ||
+        (rc = scratch_t(ctx, "j2.ch0", 1, &dch0)) || (rc = scratch_t(ctx, "j2.EE", 1, &dEE)) ||
+        (rc = scratch_t(ctx, "j2.cyc", R, &dcyc)))
+        return rc;
+    const int hs[2] = {src[srcid * 2], src[srcid * 2 + 1]};
+    const long zero = 0;
+    HIPCHK(hipMemcpyAsync(dT, T, N * (refl ? NumofR : 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    if (nrec) {
+        HIPCHK(hipMemcpyAsync(dobs, obs + ch0 * ostride, (size_t)nrec * ostride * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dn, nxyz + ch0 * 2, 2 * nrec * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(drec, rec + ch0 * 2, 2 * nrec * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(dsrc, hs, sizeof(hs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dsr, &nrec, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dch0, &zero, sizeof(long), hipMemcpyHostToDevice, st));
+    Grid2 g{nrows, ncols, dx, dz, 1.0 / (dx * dx), 1.0 / (dz * dz)};
+    if (refl)
+        HIPCHK(pfast::launch_radj2d_f64(st, dU, dU2, dp, tag, dT, dobs, Cr, drec, dn, dch0, dsr,
+                                        dsrc, 1, NumofR, MAXSwp, g, dEE, dcyc));
+    else
+        HIPCHK(pfast::launch_tadj2d_f64(st, dU, dp, tag, dT, dobs, Cr, drec, dn, dch0, dsr, dsrc,
+                                        1, MAXSwp, g, dEE, dcyc));
+    std::vector<int> cyc(R);
+    double hEE = 0;
+    HIPCHK(hipMemcpyAsync(Uout, dU, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hEE, dEE, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cyc.data(), dcyc, R * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (EE) *EE = hEE;
+    if (ncycles) std::copy(cyc.begin(), cyc.begin() + (refl ? NumofR : 1), ncycles);
+    return PFAST_OK;
+}
+
+int pfast_tadjoint_fsm2d(double *U, const double *T, const double *T_obs, double Cr,
+                         const int *rec, const double *nxyz, const int *src, const int *srnum,
+                         int srcid, int MAXSwp, double dx, double dz, int nrows, int ncols, int pe,
+                         double *EE, int *ncycles)
+{
+    (void)pe;
+    return adj2d_common(false, U, T, T_obs, Cr, rec, nxyz, src, srnum, srcid, 0, MAXSwp, dx, dz,
+                        nrows, ncols, EE, ncycles);
+}
+
+int pfast_radjoint_fsm2d(double *UR, const double *Td, const double *Tu, const double *TR_obs,
+                         double Cr, const int *rec, const double *nxyz, const int *src,
+                         const int *srnum, int srcid, int NumofR, int MAXSwp, double dx,
+                         double dz, int nrows, int ncols, int pe, double *EE, int *ncycles)
+{
+    (void)pe;
+    (void)Td;
+    if (NumofR <= 0) {
+        ARGCHK(UR, "NULL array");
+        std::fill(UR, UR + (long)nrows * ncols, 0.0);
+        if (EE) *EE = 0.0;
+        return PFAST_OK;
+    }
+    return adj2d_common(true, UR, Tu, TR_obs, Cr, rec, nxyz, src, srnum, srcid, NumofR, MAXSwp,
+                        dx, dz, nrows, ncols, EE, ncycles);
+}
+
+}  // extern "C"
+
+// ======================================================= batched 3D
+struct pfast_model3d {
+    pfast_ctx *ctx;
+    int nrows, ncols, nz;
+    double dx, dy, dz;
+    bool uniform;
+    double *vp = nullptr;  // N doubles
+    float *f = nullptr;    // N cell times / slowness
+};
+
+struct pfast_shots3d {
+    pfast_model3d *m;
+    int nsrc, nchunk;
+    long nrec;
+    std::vector<int> src;    // 3*nsrc
+    std::vector<int> srnum;  // nsrc
+    std::vector<long> ch0;   // nsrc+1 prefix
+    int *srcnode = nullptr;  // [nchunk*C]
+    float *T = nullptr, *L = nullptr;
+    // receiver table (reference order) for picks and the single-source path
+    long *rnode = nullptr;
+    int *rsrc = nullptr;
+    int *rec3 = nullptr;
+    double *nxyz = nullptr, *Tobs = nullptr;
+    // picks grouped by source (misfit, fixed-node marks)
+    long *pnode = nullptr;
+    int *psrc = nullptr;
+    double *pobs = nullptr;
+    long *poff = nullptr;
+    int npick = 0;
+    // boundary-condition entries (last picked receiver per node, not at the source)
+    long *bnode = nullptr;
+    int *bsrc = nullptr;
+    double *bobs = nullptr, *bnrm = nullptr;
+    int nb = 0;
+    double *EE = nullptr, *grad = nullptr, *picks = nullptr, *fieldbuf = nullptr;
+    unsigned *changed = nullptr;
+    int *active = nullptr;
+    unsigned *hchanged = nullptr;
+    std::vector<int> fcyc, acyc;
+    bool have_adj_data = false;
+};
+
+namespace {
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    if (hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+        pfast::set_error("hipMalloc failed (" + std::to_string(count * sizeof(T)) + " bytes)");
+        return PFAST_ERR_NOMEM;
+    }
+    return PFAST_OK;
+}
+
+template <class T>
+int dupload(T **p, const T *h, size_t count, hipStream_t st)
+{
+    int rc = dalloc(p, count);
+    if (rc) return rc;
+    if (count && h) HIPCHK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return PFAST_OK;
+}
+
+// Sweep cycles of the batched FP32 solver until every source is stationary.
+int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &cyc)
+{
+    pfast_model3d *m = s->m;
+    pfast_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const int TI = pfast::tile_ti(), TJ = pfast::tile_tj();
+    const int nti = (m->nrows + TI - 1) / TI, ntj = (m->ncols + TJ - 1) / TJ;
+    SlabArgs a{};
+    a.field = adjoint ? s->L : s->T;
+    a.tfield = s->T;
+    a.slow = m->f;
+    a.srcnode = s->srcnode;
+    a.active = s->active;
+    a.changed = s->changed;
+    a.nrows = m->nrows;
+    a.ncols = m->ncols;
+    a.nz = m->nz;
+    a.nti = nti;
+    a.ntj = ntj;
+    a.hx = (float)m->dx;
+    a.hy = (float)m->dy;
+    a.hz = (float)m->dz;
+    a.wx = (float)(1.0 / (m->dx * m->dx));
+    a.wy = (float)(1.0 / (m->dy * m->dy));
+    a.wz = (float)(1.0 / (m->dz * m->dz));
+    std::vector<int> act(s->nchunk);
+    for (int i = 0; i < s->nchunk; i++) act[i] = i;
+    cyc.assign(s->nsrc, 0);
+    int cycle = 0;
+    while (!act.empty() && cycle < MAXSwp) {
+        cycle++;
+        HIPCHK(hipMemcpyAsync(s->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
+        a.nact = (int)act.size();
+        a.nact_p8 = (a.nact + 7) / 8 * 8;
+        long real = 0;
+        for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
+        ctx->units[adjoint ? 1 : 0] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
+        for (int o = 0; o < 8; o++) {
+            const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+            for (int D = 0; D <= nti + ntj - 2; D++) {
+                const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
+                time_begin(ctx, adjoint ? 1 : 0);
+                HIPCHK(pfast::launch_sweep3d(st, a, adjoint, m->uniform, D, s1, s2, s3, hi - lo + 1));
+                time_end(ctx);
+            }
+        }
+        HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        time_collect(ctx);
+        std::vector<int> next;
+        for (int ch : act) {
+            const unsigned bits = s->hchanged[ch];
+            for (int c = 0; c < PFAST_CHUNK; c++) {
+                const int src = ch * PFAST_CHUNK + c;
+                if (src < s->nsrc && !((bits >> c) & 1u) && cyc[src] == 0) cyc[src] = cycle;
+            }
+            if (bits) next.push_back(ch);
+        }
+        act.swap(next);
+    }
+    for (int i = 0; i < s->nsrc; i++)
+        if (cyc[i] == 0) cyc[i] = cycle;
+    return PFAST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols, int nz,
+                         double dx, double dy, double dz, pfast_model3d **out)
+{
+    ARGCHK(ctx && vp && out, "NULL argument");
+    ARGCHK(nrows >= 3 && ncols >= 3 && nz >= 3, "grid dimensions must be >= 3");
+    ARGCHK(dx > 0 && dy > 0 && dz > 0, "spacings must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    auto *m = new pfast_model3d();
+    m->ctx = ctx;
+    m->nrows = nrows;
+    m->ncols = ncols;
+    m->nz = nz;
+    m->dx = dx;
+    m->dy = dy;
+    m->dz = dz;
+    m->uniform = dx == dy && dy == dz;
+    const long N = (long)nrows * ncols * nz;
+    int rc;
+    if ((rc = dalloc(&m->vp, N)) || (rc = dalloc(&m->f, N))) {
+        pfast_model3d_destroy(m);
+        return rc;
+    }
+    rc = pfast_model3d_set_vp(m, vp);
+    if (rc) {
+        pfast_model3d_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return PFAST_OK;
+}
+
+int pfast_model3d_set_vp(pfast_model3d *m, const double *vp)
+{
+    ARGCHK(m && vp, "NULL argument");
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    hipStream_t st = m->ctx->stream;
+    HIPCHK(hipMemcpyAsync(m->vp, vp, N * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(pfast::launch_cell_time(st, m->vp, m->f, N, m->dx, m->uniform));
+    HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+int pfast_model3d_destroy(pfast_model3d *m)
+{
+    if (!m) return PFAST_OK;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    if (m->vp) (void)hipFree(m->vp);
+    if (m->f) (void)hipFree(m->f);
+    delete m;
+    return PFAST_OK;
+}
+
+int pfast_shots3d_destroy(pfast_shots3d *s)
+{
+    if (!s) return PFAST_OK;
+    (void)hipStreamSynchronize(s->m->ctx->stream);
+    void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
+                    s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
+                    s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->hchanged) (void)hipHostFree(s->hchanged);
+    delete s;
+    return PFAST_OK;
+}
+
+int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
+                         const int *rec, const double *nxyz, const double *T_obs,
+                         pfast_shots3d **out)
+{
+    ARGCHK(m && src && srnum && out && nsrc > 0, "bad argument");
+    const int nrows = m->nrows, ncols = m->ncols, nz = m->nz;
+    const long N = (long)nrows * ncols * nz;
+    auto *s = new pfast_shots3d();
+    s->m = m;
+    s->nsrc = nsrc;
+    s->nchunk = (nsrc + PFAST_CHUNK - 1) / PFAST_CHUNK;
+    s->src.assign(src, src + 3 * nsrc);
+    s->srnum.assign(srnum, srnum + nsrc);
+    s->ch0.assign(nsrc + 1, 0);
+    for (int i = 0; i < nsrc; i++) {
+        ARGCHK(srnum[i] >= 0, "negative srnum");
+        s->ch0[i + 1] = s->ch0[i] + srnum[i];
+    }
+    s->nrec = s->ch0[nsrc];
+    for (int i = 0; i < nsrc; i++) {
+        const int *q = src + 3 * i;
+        if (!(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols && q[2] >= 0 && q[2] < nz)) {
+            delete s;
+            pfast::set_error("invalid argument: source node outside the grid");
+            return PFAST_ERR_ARG;
+        }
+    }
+    if (rec)
+        for (long r = 0; r < s->nrec; r++) {
+            const int *q = rec + 3 * r;
+            if (!(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols && q[2] >= 0 && q[2] < nz)) {
+                delete s;
+                pfast::set_error("invalid argument: receiver node outside the grid");
+                return PFAST_ERR_ARG;
+            }
+        }
+    hipStream_t st = m->ctx->stream;
+    std::vector<int> sn(s->nchunk * PFAST_CHUNK, -1);
+    for (int i = 0; i < nsrc; i++)
+        sn[i] = (int)(((long)src[3 * i + 2] * nrows + src[3 * i]) * ncols + src[3 * i + 1]);
+    int rc = 0;
+    const size_t fsz = (size_t)s->nchunk * N * PFAST_CHUNK;
+    if ((rc = dupload(&s->srcnode, sn.data(), sn.size(), st)) || (rc = dalloc(&s->T, fsz)) ||
+        (rc = dalloc(&s->changed, s->nchunk)) || (rc = dalloc(&s->active, s->nchunk)) ||
+        (rc = dalloc(&s->EE, nsrc)) || (rc = dalloc(&s->grad, N)) ||
+        (rc = dalloc(&s->fieldbuf, N))) {
+        pfast_shots3d_destroy(s);
+        return rc;
+    }
+    if (hipHostMalloc((void **)&s->hchanged, s->nchunk * sizeof(unsigned)) != hipSuccess) {
+        pfast_shots3d_destroy(s);
+        pfast::set_error("hipHostMalloc failed");
+        return PFAST_ERR_NOMEM;
+    }
+    // receiver table: node index and owning source of each entry
+    std::vector<long> rnode(s->nrec);
+    std::vector<int> rsrc(s->nrec);
+    if (rec) {
+        for (int i = 0; i < nsrc; i++)
+            for (long r = s->ch0[i]; r < s->ch0[i + 1]; r++) {
+                const int *q = rec + 3 * r;
+                rnode[r] = ((long)q[2] * nrows + q[0]) * ncols + q[1];
+                rsrc[r] = i;
+            }
+        if ((rc = dupload(&s->rnode, rnode.data(), s->nrec, st)) ||
+            (rc = dupload(&s->rsrc, rsrc.data(), s->nrec, st)) ||
+            (rc = dupload(&s->rec3, rec, 3 * s->nrec, st)) || (rc = dalloc(&s->picks, s->nrec))) {
+            pfast_shots3d_destroy(s);
+            return rc;
+        }
+    }
+    if (rec && T_obs && nxyz) {
+        s->have_adj_data = true;
+        std::vector<long> pnode, bnode;
+        std::vector<int> psrc, bsrc;
+        std::vector<double> pobs, bobs, bnrm;
+        std::vector<long> poff(nsrc + 1, 0);
+        for (int i = 0; i < nsrc; i++) {
+            const long sn0 = sn[i];
+            std::map<long, long> last;  // node -> last picked receiver (reference overwrite order)
+            for (long r = s->ch0[i]; r < s->ch0[i + 1]; r++) {
+                if (!(T_obs[r] < 100)) continue;
+                pnode.push_back(rnode[r]);
+                psrc.push_back(i);
+                pobs.push_back(T_obs[r]);
+                last[rnode[r]] = r;
+            }
+            poff[i + 1] = (long)pnode.size();
+            for (auto &kv : last) {
+                if (kv.first == sn0) continue;  // receiver at the source: lambda stays 0
+                const long r = kv.second;
+                bnode.push_back(kv.first);
+                bsrc.push_back(i);
+                bobs.push_back(T_obs[r]);
+                bnrm.insert(bnrm.end(), nxyz + 3 * r, nxyz + 3 * r + 3);
+            }
+        }
+        s->npick = (int)pnode.size();
+        s->nb = (int)bnode.size();
+        if ((rc = dalloc(&s->L, fsz)) || (rc = dupload(&s->pnode, pnode.data(), pnode.size(), st)) ||
+            (rc = dupload(&s->psrc, psrc.data(), psrc.size(), st)) ||
+            (rc = dupload(&s->pobs, pobs.data(), pobs.size(), st)) ||
+            (rc = dupload(&s->poff, poff.data(), poff.size(), st)) ||
+            (rc = dupload(&s->bnode, bnode.data(), bnode.size(), st)) ||
+            (rc = dupload(&s->bsrc, bsrc.data(), bsrc.size(), st)) ||
+            (rc = dupload(&s->bobs, bobs.data(), bobs.size(), st)) ||
+            (rc = dupload(&s->bnrm, bnrm.data(), bnrm.size(), st)) ||
+            (rc = dupload(&s->nxyz, nxyz, 3 * s->nrec, st)) ||
+            (rc = dupload(&s->Tobs, T_obs, s->nrec, st))) {
+            pfast_shots3d_destroy(s);
+            return rc;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    *out = s;
+    return PFAST_OK;
+}
+
+int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp)
+{
+    ARGCHK(s, "NULL shots");
+    ARGCHK(mode == PFAST_FWD_ONLY || mode == PFAST_FWD_ADJ, "mode");
+    ARGCHK(mode == PFAST_FWD_ONLY || s->have_adj_data, "shot set has no picks for the adjoint");
+    ARGCHK(MAXSwp >= 0, "MAXSwp < 0");
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    HIPCHK(hipSetDevice(m->ctx->device));
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    HIPCHK(pfast::launch_init_fwd(st, s->T, s->srcnode, N, s->nchunk));
+    int rc = run_sweeps3d(s, false, MAXSwp, s->fcyc);
+    if (rc) return rc;
+    if (mode == PFAST_FWD_ONLY) return PFAST_OK;
+    HIPCHK(pfast::launch_adj_misfit(st, s->T, s->pnode, s->pobs, s->poff, s->nsrc, N, s->EE));
+    HIPCHK(hipMemsetAsync(s->L, 0, (size_t)s->nchunk * N * PFAST_CHUNK * sizeof(float), st));
+    HIPCHK(pfast::launch_adj_bc(st, s->L, s->T, s->bnode, s->bsrc, s->bobs, s->bnrm, s->nb,
+                                m->nrows, m->ncols, m->nz, m->dx, m->dy, m->dz));
+    HIPCHK(pfast::launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 0));
+    rc = run_sweeps3d(s, true, MAXSwp, s->acyc);
+    if (rc) return rc;
+    HIPCHK(pfast::launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 1));
+    HIPCHK(pfast::launch_grad_reduce(st, s->L, m->vp, s->grad, N, s->nchunk, s->nsrc));
+    HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+int pfast_shots3d_gradient(pfast_shots3d *s, double *grad, double *misfit)
+{
+    ARGCHK(s && s->have_adj_data, "no adjoint results");
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    if (grad) HIPCHK(hipMemcpyAsync(grad, s->grad, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> ee(s->nsrc);
+    HIPCHK(hipMemcpyAsync(ee.data(), s->EE, s->nsrc * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (misfit) {
+        double sum = 0, comp = 0;  // Kahan over sources
+        for (double e : ee) {
+            const double y = e - comp, t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+        *misfit = sum;
+    }
+    return PFAST_OK;
+}
+
+int pfast_shots3d_picks(pfast_shots3d *s, double *picks)
+{
+    ARGCHK(s && picks && s->rnode, "no receiver table");
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    HIPCHK(pfast::launch_gather_picks(st, s->T, s->rnode, s->rsrc, (int)s->nrec, N, s->picks));
+    HIPCHK(hipMemcpyAsync(picks, s->picks, s->nrec * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field)
+{
+    ARGCHK(s && field && src >= 0 && src < s->nsrc && (which == 0 || which == 1), "bad argument");
+    ARGCHK(which == 0 || s->L, "no adjoint field");
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, s->fieldbuf));
+    HIPCHK(hipMemcpyAsync(field, s->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+int pfast_shots3d_cycles(pfast_shots3d *s, int *cycles)
+{
+    ARGCHK(s && cycles, "bad argument");
+    for (int i = 0; i < s->nsrc; i++) {
+        cycles[2 * i] = i < (int)s->fcyc.size() ? s->fcyc[i] : 0;
+        cycles[2 * i + 1] = i < (int)s->acyc.size() ? s->acyc[i] : 0;
+    }
+    return PFAST_OK;
+}
+
+// Single-source FP64 forward+adjoint of source `srcid` of a resident shot set:
+// the reference's per-source loop body (src/T_PFAST3D.cpp:388-393) on the GPU,
+// on the same resident model.  grad (device, N doubles) accumulates -lambda/v^3.
+int pfast_shots3d_plain_source(pfast_shots3d *s, int srcid, int MAXSwp, double *EE,
+                               int *cycles2, double *T_out, double *lam_out)
+{
+    ARGCHK(s && s->have_adj_data && srcid >= 0 && srcid < s->nsrc, "bad argument");
+    pfast_model3d *m = s->m;
+    pfast_ctx *ctx = m->ctx;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    double *T, *U, *p, *dEE;
+    int *flag;
+    unsigned char *tag;
+    int rc;
+    if ((rc = scratch_t(ctx, "ps.T", N, &T)) || (rc = scratch_t(ctx, "ps.U", N, &U)) ||
+        (rc = scratch_t(ctx, "ps.p", N, &p)) || (rc = scratch_t(ctx, "ps.tag", N, &tag)) ||
+        (rc = scratch_t(ctx, "ps.EE", 1, &dEE)) || (rc = scratch_t(ctx, "ps.flag", 1, &flag)))
+        return rc;
+    const int si = s->src[3 * srcid], sj = s->src[3 * srcid + 1], sk = s->src[3 * srcid + 2];
+    int fc = 0, ac = 0;
+    rc = fsm3d_dev(ctx, m->vp, T, p, flag, si, sj, sk, MAXSwp, m->dx, m->dy, m->dz, m->nrows,
+                   m->ncols, m->nz, &fc);
+    if (rc) return rc;
+    const long c0 = s->ch0[srcid];
+    rc = adj3d_dev(ctx, U, T, tag, p, flag, dEE, s->Tobs + c0, s->rec3 + 3 * c0, s->nxyz + 3 * c0,
+                   s->srnum[srcid], si, sj, sk, MAXSwp, m->dx, m->dy, m->dz, m->nrows, m->ncols,
+                   m->nz, EE, &ac);
+    if (rc) return rc;
+    if (cycles2) {
+        cycles2[0] = fc;
+        cycles2[1] = ac;
+    }
+    if (T_out) HIPCHK(hipMemcpyAsync(T_out, T, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (lam_out) HIPCHK(hipMemcpyAsync(lam_out, U, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return PFAST_OK;
+}
+
+}  // extern "C"

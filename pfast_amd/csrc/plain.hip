@@ -1,0 +1,682 @@
+// plain.hip -- single-source FP64 kernels (bit-identical to the reference's FP64).
+//
+// 3D: one launch per anti-diagonal hyperplane of an ordering.  In ordering
+// (s3,s1,s2) the reference visits nodes lexicographically; a node's update reads
+// only its six face neighbours, and both in lexicographic and in hyperplane order
+// every face neighbour "behind" it (in each axis) is already updated in this
+// ordering and every neighbour "ahead" is not.  Nodes of one hyperplane are never
+// face neighbours, so a hyperplane is one parallel step and the results are the
+// reference's bit for bit.
+//
+// 2D: one work-group per source runs the whole solve (all cycles, orderings and
+// hyperplanes, with work-group barriers between hyperplanes), for both the FP64
+// reference arithmetic and the FP32 batched form.
+#include "pfast_internal.h"
+#include "plain_kernels.h"
+#include "solvers.h"
+
+// --------------------------------------------------------------------- 3D FP64
+__global__ void init3d_f64(double *U, long N, long srcnode)
+{
+    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) U[n] = n == srcnode ? 0.0 : PFAST_INF_D;
+}
+
+// Plane d of ordering (s3,s1,s2): transformed coords kk,ii,jj with kk+ii+jj = d.
+// Thread t covers (kk, ii) in [k0,k0+nk) x [i0,i0+ni); jj is implied.
+__global__ void fsm3d_plane_f64(const double *__restrict__ vp, double *__restrict__ U,
+                                double *__restrict__ p, int *__restrict__ flag, int si, int sj,
+                                int sk, int s1, int s2, int s3, int d, int k0, int nk, int i0,
+                                int ni, double dx, double dy, double dz, int nrows, int ncols,
+                                int nz, int first, int last)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * ni) return;
+    int kk = k0 + t / ni, ii = i0 + t % ni, jj = d - kk - ii;
+    if (jj < 0 || jj >= ncols) return;
+    int k = s3 > 0 ? kk : nz - 1 - kk, i = s1 > 0 ? ii : nrows - 1 - ii,
+        j = s2 > 0 ? jj : ncols - 1 - jj;
+    if (i == si && j == sj && k == sk) return;
+    const long plane = (long)nrows * ncols;
+    const long n = k * plane + (long)i * ncols + j;
+    double a, b, c;
+    if (j - 1 < 0) a = U[n + 1];
+    else if (j + 1 >= ncols) a = U[n - 1];
+    else { double x1 = U[n + 1], x0 = U[n - 1]; a = x1 < x0 ? x1 : x0; }
+    if (i - 1 < 0) b = U[n + ncols];
+    else if (i + 1 >= nrows) b = U[n - ncols];
+    else { double y0 = U[n - ncols], y1 = U[n + ncols]; b = y0 < y1 ? y0 : y1; }
+    if (k - 1 < 0) c = U[n + plane];
+    else if (k + 1 >= nz) c = U[n - plane];
+    else { double z0 = U[n - plane], z1 = U[n + plane]; c = z0 < z1 ? z0 : z1; }
+    const double s = 1.0 / vp[n];
+    const double q = eik3d_ref(a, b, c, s, dx, dy, dz);
+    const double old = U[n];
+    const double nw = old < q ? old : q;
+    U[n] = nw;
+    if (first) p[n] = old;
+    if (last && fabs(p[n] - nw) > 1e-9) *flag = 1;
+}
+
+// Receiver boundary condition of src/T_padjoint3d.h:90-127 for one source, in
+// receiver order (one thread: duplicates overwrite and EE accumulates exactly as
+// the reference does).  Also computes the first convergence test isconv(0, U).
+__global__ void adj3d_bc_f64(double *__restrict__ U, unsigned char *__restrict__ tag,
+                             const double *__restrict__ T, const double *__restrict__ T_obs,
+                             const int *__restrict__ rec, const double *__restrict__ nxyz,
+                             long ch0, int nrec, int si, int sj, int sk, double dx, double dy,
+                             double dz, int nrows, int ncols, int nz, double *__restrict__ EE_out,
+                             int *__restrict__ flag)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const long plane = (long)nrows * ncols;
+    double EE = 0.0;
+    for (int idx = 0; idx < nrec; idx++) {
+        const long r = idx + ch0;
+        if (!(T_obs[r] < 100)) continue;
+        const int i = rec[r * 3 + 0], j = rec[r * 3 + 1], k = rec[r * 3 + 2];
+        const long n = k * plane + (long)i * ncols + j;
+        tag[n] = 1;
+        EE += (T_obs[r] - T[n]) * (T_obs[r] - T[n]) / 2.0;
+        if (i == si && j == sj && k == sk) continue;
+        const double nny = nxyz[r * 3 + 0], nnx = nxyz[r * 3 + 1], nnz = nxyz[r * 3 + 2];
+        double Tx, Ty, Tz;
+        if (k + 1 >= nz) Tz = (T[n] - T[n - plane]) / dz;
+        else Tz = (T[n + plane] - T[n]) / dz;
+        if (i + 1 >= nrows) Ty = (T[n] - T[n - ncols]) / dy;
+        else Ty = (T[n + ncols] - T[n]) / dy;
+        if (j + 1 >= ncols) Tx = (T[n] - T[n - 1]) / dx;
+        else Tx = (T[n + 1] - T[n]) / dx;
+        if (fabs(T_obs[r] - T[n]) > 1e-9 && fabs(nnx * Tx + nny * Ty + nnz * Tz) > 1e-6)
+            U[n] = (T_obs[r] - T[n]) / (nnx * Tx + nny * Ty + nnz * Tz);
+        else
+            U[n] = 0.0;
+    }
+    int nc = 0;
+    for (int idx = 0; idx < nrec && !nc; idx++) {
+        const long r = idx + ch0;
+        if (!(T_obs[r] < 100)) continue;
+        const long n = rec[r * 3 + 2] * plane + (long)rec[r * 3 + 0] * ncols + rec[r * 3 + 1];
+        if (fabs(U[n]) > 1e-9) nc = 1;
+    }
+    *EE_out = EE;
+    *flag = nc;
+}
+
+// Adjoint hyperplane over the interior box [1,n-2]^3 (src/T_padjoint3d.h:143-155).
+__global__ void adj3d_plane_f64(double *__restrict__ U, const double *__restrict__ T,
+                                const unsigned char *__restrict__ tag, double *__restrict__ p,
+                                int *__restrict__ flag, int s1, int s2, int s3, int d, int k0,
+                                int nk, int i0, int ni, double dx, double dy, double dz,
+                                int nrows, int ncols, int nz, int first, int last)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * ni) return;
+    const int nci = ncols - 2;
+    int kk = k0 + t / ni, ii = i0 + t % ni, jj = d - kk - ii;
+    if (jj < 0 || jj >= nci) return;
+    int k = s3 > 0 ? 1 + kk : nz - 2 - kk, i = s1 > 0 ? 1 + ii : nrows - 2 - ii,
+        j = s2 > 0 ? 1 + jj : ncols - 2 - jj;
+    const long plane = (long)nrows * ncols;
+    const long n = k * plane + (long)i * ncols + j;
+    if (tag[n]) return;
+    const double old = U[n];
+    const double nw = adj3d_ref(T[n], T[n + 1], T[n - 1], T[n + ncols], T[n - ncols],
+                                T[n + plane], T[n - plane], U[n + 1], U[n - 1], U[n + ncols],
+                                U[n - ncols], U[n + plane], U[n - plane], dx, dy, dz);
+    U[n] = nw;
+    if (first) p[n] = old;
+    if (last && fabs(p[n] - nw) > 1e-9) *flag = 1;
+}
+
+// -------------------------------------------------------------------------- 2D
+// Policies: Ref64 = the reference's FP64 arithmetic and its |p-U| <= 1e-9 test;
+// Fast32 = FP32 difference-form updates with a bitwise-stationarity test.
+struct Ref64 {
+    using real = double;
+    static constexpr bool exact = true;
+    __device__ static real eik(real a, real b, real s, const Grid2 &g)
+    {
+        return eik2d_ref(a, b, s, g.dx, g.dz);
+    }
+    __device__ static real adj(real tc, real txp, real txm, real tzp, real tzm, real lxp,
+                               real lxm, real lzp, real lzm, const Grid2 &g)
+    {
+        return adj2d_ref(tc, txp, txm, tzp, tzm, lxp, lxm, lzp, lzm, g.dx, g.dz);
+    }
+    __device__ static real slow(const double *vp, long n) { return 1.0 / vp[n]; }
+};
+
+struct Fast32 {
+    using real = float;
+    static constexpr bool exact = false;
+    __device__ static real eik(real a, real b, real s, const Grid2 &g)
+    {
+        return eik2d_f32(a, b, s, (float)g.dx, (float)g.dz, (float)g.wx, (float)g.wz);
+    }
+    __device__ static real adj(real tc, real txp, real txm, real tzp, real tzm, real lxp,
+                               real lxm, real lzp, real lzm, const Grid2 &g)
+    {
+        const float I = PFAST_INF_F;
+        if (tc >= I || txp >= I || txm >= I || tzp >= I || tzm >= I) return 0.0f;
+        float den = 0.0f;
+        float num = adj_term(tc, txp, lxp, (float)g.wx, den);
+        num += adj_term(tc, txm, lxm, (float)g.wx, den);
+        num += adj_term(tc, tzp, lzp, (float)g.wz, den);
+        num += adj_term(tc, tzm, lzm, (float)g.wz, den);
+        return fabsf(den) > 1e-9f ? num / den : 0.0f;
+    }
+    __device__ static real slow(const double *vp, long n) { return (float)(1.0 / vp[n]); }
+};
+
+// One 2D FSM solve (all cycles) by the calling work-group.  mode 0: all nodes but
+// the source; 1: also only (i+1)*dz <= Z depth of column j (down-going,
+// src/RT_pfsm2d.h:174); 2: only i <= k2vec[j] (up-going, :309).
+// Returns the cycles swept (reference `count` capped at MAXSwp).
+template <class P>
+__device__ int wg_fsm2d(const double *__restrict__ vp, typename P::real *__restrict__ U,
+                        typename P::real *__restrict__ pbuf, int si, int sj, int mode,
+                        const double *__restrict__ Zr, const int *k2vec, int MAXSwp,
+                        const Grid2 &g, int *s_flag)
+{
+    using real = typename P::real;
+    const int nrows = g.nrows, ncols = g.ncols;
+    const int nplanes = nrows + ncols - 1;
+    int count = 0;
+    for (;;) {
+        if (!(count < MAXSwp)) break;
+        count++;
+        if (threadIdx.x == 0) *s_flag = 0;
+        __syncthreads();
+        for (int o = 0; o < 4; o++) {
+            const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
+            const bool first = o == 0, last = o == 3;
+            for (int d = 0; d < nplanes; d++) {
+                const int ilo = d - (ncols - 1) > 0 ? d - (ncols - 1) : 0;
+                const int ihi = d < nrows - 1 ? d : nrows - 1;
+                for (int ii = ilo + (int)threadIdx.x; ii <= ihi; ii += blockDim.x) {
+                    const int jj = d - ii;
+                    const int i = s1 > 0 ? ii : nrows - 1 - ii, j = s2 > 0 ? jj : ncols - 1 - jj;
+                    bool upd;
+                    if (mode == 0) upd = !(i == si && j == sj);
+                    else if (mode == 1) upd = !(i == si && j == sj) && (i + 1) * g.dz <= Zr[j * 3];
+                    else upd = i <= k2vec[j];
+                    if (!upd) continue;
+                    const long n = (long)i * ncols + j;
+                    real a, b;
+                    if (i - 1 < 0) b = U[n + ncols];
+                    else if (i + 1 >= nrows) b = U[n - ncols];
+                    else { real u0 = U[n - ncols], u1 = U[n + ncols]; b = u0 < u1 ? u0 : u1; }
+                    if (j - 1 < 0) a = U[n + 1];
+                    else if (j + 1 >= ncols) a = U[n - 1];
+                    else { real u1 = U[n + 1], u0 = U[n - 1]; a = u1 < u0 ? u1 : u0; }
+                    const real q = P::eik(a, b, P::slow(vp, n), g);
+                    const real old = U[n];
+                    const real nw = old < q ? old : q;
+                    U[n] = nw;
+                    if (P::exact) {
+                        if (first) pbuf[n] = old;
+                        if (last && fabs((double)pbuf[n] - (double)nw) > 1e-9) *s_flag = 1;
+                    } else if (nw != old) {
+                        *s_flag = 1;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const int changed = *s_flag;
+        __syncthreads();
+        if (!changed) break;
+    }
+    return count;
+}
+
+template <class P>
+__device__ double reflect_seed_dev(const typename P::real *U, const double *Zr, int jj, int k2,
+                                   double qx, double dz, int ncols)
+{
+    // src/RT_pfsm2d.h:220-223 (FP64 arithmetic on the stored times)
+    const double u0 = (double)U[(long)k2 * ncols + jj], u1 = (double)U[(long)(k2 + 1) * ncols + jj];
+    double qz = (u1 - u0) / dz;
+    double dnx = Zr[jj * 3 + 2], dnz = Zr[jj * 3 + 1];
+    double rz = qz - 2 * (qx * dnx + qz * dnz) * dnz;
+    return u0 + qz * (Zr[jj * 3] - k2 * dz - dz) + rz * ((k2 + 1) * dz + dz - Zr[jj * 3]);
+}
+
+// Whole RT_fsm2d (src/RT_pfsm2d.h:116-380) for source blockIdx.x of a batch.
+// Fields are per-source contiguous: U [b][N], TTd/TTu [b][NumofR][N].
+template <class P>
+__global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp,
+                                                   typename P::real *__restrict__ Uall,
+                                                   typename P::real *__restrict__ TTd,
+                                                   typename P::real *__restrict__ TTu,
+                                                   typename P::real *__restrict__ pall,
+                                                   const double *__restrict__ Z,
+                                                   const int *__restrict__ src, int NumofR,
+                                                   int MAXSwp, Grid2 g, int *__restrict__ cycles,
+                                                   int *__restrict__ status,
+                                                   int *__restrict__ k2out)
+{
+    using real = typename P::real;
+    extern __shared__ int s_k2[];  // ncols ints (+1 flag)
+    int *s_flag = s_k2 + g.ncols;
+    const int b = blockIdx.x;
+    const int nrows = g.nrows, ncols = g.ncols;
+    const long N = (long)nrows * ncols;
+    real *U = Uall + b * N;
+    real *pbuf = pall ? pall + b * N : nullptr;
+    const int si = src[b * 2 + 0], sj = src[b * 2 + 1];
+    const double dz = g.dz, dx = g.dx;
+    int st = 0;
+    for (int j = threadIdx.x; j < ncols; j += blockDim.x) s_k2[j] = -1;
+    __syncthreads();
+    for (int ir = 0; ir < NumofR && st == 0; ir++) {
+        const double *Zr = Z + (long)ir * ncols * 3;
+        for (long n = threadIdx.x; n < N; n += blockDim.x) {
+            U[n] = PFAST_INF_D;
+            if (pbuf) pbuf[n] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) U[(long)si * ncols + sj] = 0;
+        __syncthreads();
+        int c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 1, Zr, s_k2, MAXSwp, g, s_flag);
+        if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR) + 1 + 2 * ir] = c;
+        real *Td = TTd + ((long)b * NumofR + ir) * N;
+        for (long n = threadIdx.x; n < N; n += blockDim.x) Td[n] = U[n];
+        __syncthreads();
+        // reflection points and seeds (src/RT_pfsm2d.h:195-281): sequential in the
+        // reference's column order because a seed can feed the next column's qx.
+        if (threadIdx.x == 0) {
+            for (int step = 0; step < ncols && st == 0; step++) {
+                // columns 1..ncols-2, then 0, then ncols-1
+                const int jj = step < ncols - 2 ? step + 1 : (step == ncols - 2 ? 0 : ncols - 1);
+                const bool left = jj == 0, right = jj == ncols - 1;
+                int k2 = -1;
+                for (int kk = 0; kk < nrows - 1; kk++) {
+                    bool cnd;
+                    if (left) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj + 1) * 3];
+                    else if (right) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj - 1) * 3];
+                    else cnd = (kk + 1) * dz + dz <= Zr[jj * 3] &&
+                               (kk * dz + dz <= Zr[(jj - 1) * 3] || kk * dz + dz <= Zr[(jj + 1) * 3]);
+                    if (cnd && kk >= k2) k2 = kk;
+                }
+                if (Zr[jj * 3] <= nrows * dz && k2 > 0) {
+                    const long r0 = (long)k2 * ncols;
+                    double qx;
+                    if (left) qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj]) / dx;
+                    else if (right) qx = ((double)U[r0 + jj] - (double)U[r0 + jj - 1]) / dx;
+                    else if (k2 * dz + dz <= Zr[(jj - 1) * 3] && k2 * dz + dz <= Zr[(jj + 1) * 3])
+                        qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj - 1]) / 2.0 / dx;
+                    else if (k2 * dz + dz <= Zr[(jj - 1) * 3])
+                        qx = ((double)U[r0 + jj] - (double)U[r0 + jj - 1]) / dx;
+                    else
+                        qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj]) / dx;
+                    const double v = reflect_seed_dev<P>(U, Zr, jj, k2, qx, dz, ncols);
+                    U[(long)(k2 + 1) * ncols + jj] = (real)v;
+                    s_k2[jj] = k2;
+                    if (v < 0) st = 2;
+                } else if (Zr[jj * 3] > nrows * dz) {
+                    s_k2[jj] = nrows - 1;
+                } else {
+                    st = 1;
+                }
+            }
+            *s_flag = st;
+        }
+        __syncthreads();
+        st = *s_flag;
+        __syncthreads();
+        if (st) break;
+        // reset above the reflection points (src/RT_pfsm2d.h:289-295)
+        for (long n = threadIdx.x; n < N; n += blockDim.x) {
+            const int i = (int)(n / ncols), j = (int)(n % ncols);
+            if (i <= s_k2[j]) U[n] = PFAST_INF_D;
+            if (pbuf) pbuf[n] = 0;
+        }
+        __syncthreads();
+        c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 2, Zr, s_k2, MAXSwp, g, s_flag);
+        if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR) + 2 + 2 * ir] = c;
+        real *Tu = TTu + ((long)b * NumofR + ir) * N;
+        for (long n = threadIdx.x; n < N; n += blockDim.x) Tu[n] = U[n];
+        __syncthreads();
+    }
+    if (st == 0) {
+        for (long n = threadIdx.x; n < N; n += blockDim.x) {
+            U[n] = PFAST_INF_D;
+            if (pbuf) pbuf[n] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) U[(long)si * ncols + sj] = 0;
+        __syncthreads();
+        int c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 0, Z, s_k2, MAXSwp, g, s_flag);
+        if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR)] = c;
+    }
+    if (k2out)
+        for (int j = threadIdx.x; j < ncols; j += blockDim.x) k2out[(long)b * ncols + j] = s_k2[j];
+    if (threadIdx.x == 0 && status) status[b] = st;
+}
+
+// Interior adjoint sweeps (src/RT_padjoint2d.h:124-148) with the reference's
+// carried `count` (Radjoint_fsm2d does not reset it between reflectors).
+// first_nc: result of the first convergence test isconv(0, U) negated.
+template <class P>
+__device__ int wg_adj2d(typename P::real *__restrict__ U, typename P::real *__restrict__ pbuf,
+                        const typename P::real *__restrict__ T,
+                        const unsigned char *__restrict__ tag, int &count, int first_nc,
+                        int MAXSwp, const Grid2 &g, int *s_flag)
+{
+    using real = typename P::real;
+    const int nrows = g.nrows, ncols = g.ncols;
+    const int nri = nrows - 2, nci = ncols - 2;
+    const int nplanes = nri + nci - 1;
+    int notconv = first_nc, swept = 0;
+    for (;;) {
+        if (!(count < MAXSwp)) { count++; break; }
+        count++;
+        if (!notconv) break;
+        swept++;
+        if (threadIdx.x == 0) *s_flag = 0;
+        __syncthreads();
+        for (int o = 0; o < 4; o++) {
+            const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
+            const bool first = o == 0, last = o == 3;
+            for (int d = 0; d < nplanes; d++) {
+                const int ilo = d - (nci - 1) > 0 ? d - (nci - 1) : 0;
+                const int ihi = d < nri - 1 ? d : nri - 1;
+                for (int ii = ilo + (int)threadIdx.x; ii <= ihi; ii += blockDim.x) {
+                    const int jj = d - ii;
+                    const int i = s1 > 0 ? 1 + ii : nrows - 2 - ii, j = s2 > 0 ? 1 + jj : ncols - 2 - jj;
+                    const long n = (long)i * ncols + j;
+                    if (tag[n]) continue;
+                    const real old = U[n];
+                    const real nw = P::adj(T[n], T[n + 1], T[n - 1], T[n + ncols], T[n - ncols],
+                                           U[n + 1], U[n - 1], U[n + ncols], U[n - ncols], g);
+                    U[n] = nw;
+                    if (P::exact) {
+                        if (first) pbuf[n] = old;
+                        if (last && fabs((double)pbuf[n] - (double)nw) > 1e-9) *s_flag = 1;
+                    } else if (nw != old) {
+                        *s_flag = 1;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        notconv = *s_flag;
+        __syncthreads();
+    }
+    return swept;
+}
+
+// Receiver BC for one source (thread 0 of the block), src/RT_padjoint2d.h:298-330
+// (transmission, checkSrc=1, pick T_obs<100) and :402-435 (reflection, checkSrc=0,
+// pick TR_obs != 0).  Adds to the running EE in receiver order, as the reference
+// does; sets *first_nc to !isconv(0,U).
+template <class P>
+__device__ void bc2d(typename P::real *U, unsigned char *tag, const typename P::real *T,
+                     const double *obs, int obs_stride, int obs_off, bool transmission,
+                     double Cr, const int *rec, const double *nxy, long ch0, int nrec, int si,
+                     int sj, const Grid2 &g, int *first_nc, double &EE)
+{
+    const int nrows = g.nrows, ncols = g.ncols;
+    for (int idx = 0; idx < nrec; idx++) {
+        const long r = idx + ch0;
+        const double o = obs[r * obs_stride + obs_off];
+        if (transmission ? !(o < 100) : !(o != 0)) continue;
+        const int i = rec[r * 2 + 0], j = rec[r * 2 + 1];
+        const long n = (long)i * ncols + j;
+        tag[n] = 1;
+        const double Tn = (double)T[n];
+        EE += Cr * (o - Tn) * (o - Tn) / 2.0;
+        if (transmission && i == si && j == sj) continue;
+        const double nny = nxy[r * 2 + 0], nnx = nxy[r * 2 + 1];
+        double Tx, Tz;
+        if (i + 1 >= nrows) Tz = (Tn - (double)T[n - ncols]) / g.dz;
+        else Tz = ((double)T[n + ncols] - Tn) / g.dz;
+        if (j + 1 >= ncols) Tx = (Tn - (double)T[n - 1]) / g.dx;
+        else Tx = ((double)T[n + 1] - Tn) / g.dx;
+        if (fabs(o - Tn) > 1e-9 && fabs(nnx * Tx + nny * Tz) > 1e-6)
+            U[n] = (typename P::real)(Cr * (o - Tn) / (nnx * Tx + nny * Tz));
+        else
+            U[n] = 0;
+    }
+    int nc = 0;
+    for (int idx = 0; idx < nrec && !nc; idx++) {
+        const long r = idx + ch0;
+        const double o = obs[r * obs_stride + obs_off];
+        if (transmission ? !(o < 100) : !(o != 0)) continue;
+        const long n = (long)rec[r * 2 + 0] * ncols + rec[r * 2 + 1];
+        if (fabs((double)U[n]) > 1e-9) nc = 1;
+    }
+    *first_nc = nc;
+}
+
+// Tadjoint_fsm2d (src/RT_padjoint2d.h:288-380, Cr-weighted overload :67-157) for
+// source `b = blockIdx.x` of a batch: U [b][N], tag scratch [b][N].
+template <class P>
+__global__ __launch_bounds__(256) void tadj2d_wg(typename P::real *__restrict__ Uall,
+                                                 typename P::real *__restrict__ pall,
+                                                 unsigned char *__restrict__ tagall,
+                                                 const typename P::real *__restrict__ Tall,
+                                                 const double *__restrict__ T_obs, double Cr,
+                                                 const int *__restrict__ rec,
+                                                 const double *__restrict__ nxy,
+                                                 const long *__restrict__ ch0,
+                                                 const int *__restrict__ srnum,
+                                                 const int *__restrict__ src, int MAXSwp, Grid2 g,
+                                                 double *__restrict__ EE, int *__restrict__ cycles)
+{
+    __shared__ int s_flag[2];
+    const int b = blockIdx.x;
+    const long N = (long)g.nrows * g.ncols;
+    auto *U = Uall + b * N;
+    auto *pbuf = pall ? pall + b * N : nullptr;
+    auto *tag = tagall + b * N;
+    const auto *T = Tall + b * N;
+    for (long n = threadIdx.x; n < N; n += blockDim.x) {
+        U[n] = 0;
+        tag[n] = 0;
+        if (pbuf) pbuf[n] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int nc = 0;
+        double e = 0.0;
+        bc2d<P>(U, tag, T, T_obs, 1, 0, true, Cr, rec, nxy, ch0[b], srnum[b], src[b * 2],
+                src[b * 2 + 1], g, &nc, e);
+        EE[b] = e;
+        s_flag[1] = nc;
+    }
+    __syncthreads();
+    int count = 0;
+    const int nc = s_flag[1];
+    int sw = wg_adj2d<P>(U, pbuf, T, tag, count, nc, MAXSwp, g, s_flag);
+    if (threadIdx.x == 0 && cycles) cycles[b] = sw;
+}
+
+// Radjoint_fsm2d (src/RT_padjoint2d.h:382-473, Cr-weighted overload :159-285).
+// Tu [b][NumofR][N]; UR [b][N]; U/p/tag scratch [b][N].
+template <class P>
+__global__ __launch_bounds__(256) void radj2d_wg(typename P::real *__restrict__ URall,
+                                                 typename P::real *__restrict__ Uall,
+                                                 typename P::real *__restrict__ pall,
+                                                 unsigned char *__restrict__ tagall,
+                                                 const typename P::real *__restrict__ Tuall,
+                                                 const double *__restrict__ TR_obs, double Cr,
+                                                 const int *__restrict__ rec,
+                                                 const double *__restrict__ nxy,
+                                                 const long *__restrict__ ch0,
+                                                 const int *__restrict__ srnum,
+                                                 const int *__restrict__ src, int NumofR,
+                                                 int MAXSwp, Grid2 g, double *__restrict__ EE,
+                                                 int *__restrict__ cycles)
+{
+    __shared__ int s_flag[2];
+    const int b = blockIdx.x;
+    const long N = (long)g.nrows * g.ncols;
+    auto *UR = URall + b * N;
+    auto *U = Uall + b * N;
+    auto *pbuf = pall ? pall + b * N : nullptr;
+    auto *tag = tagall + b * N;
+    for (long n = threadIdx.x; n < N; n += blockDim.x) {
+        UR[n] = 0;
+        tag[n] = 0;
+    }
+    int count = 0;
+    double EEacc = 0.0;
+    for (int ir = 0; ir < NumofR; ir++) {
+        const auto *T = Tuall + ((long)b * NumofR + ir) * N;
+        for (long n = threadIdx.x; n < N; n += blockDim.x) {
+            U[n] = 0;
+            if (pbuf) pbuf[n] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int nc = 0;
+            bc2d<P>(U, tag, T, TR_obs, NumofR, ir, false, Cr, rec, nxy, ch0[b], srnum[b],
+                    src[b * 2], src[b * 2 + 1], g, &nc, EEacc);
+            s_flag[1] = nc;
+        }
+        __syncthreads();
+        const int nc = s_flag[1];
+        int sw = wg_adj2d<P>(U, pbuf, T, tag, count, nc, MAXSwp, g, s_flag);
+        if (threadIdx.x == 0 && cycles) cycles[b * NumofR + ir] = sw;
+        for (long n = threadIdx.x; n < N; n += blockDim.x) UR[n] = UR[n] + U[n];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) EE[b] = EEacc;
+}
+
+// ------------------------------------------------------------ launch wrappers
+namespace pfast {
+
+hipError_t launch_rt_fsm2d_f64(hipStream_t st, const double *vp, double *U, double *TTd,
+                               double *TTu, double *p, const double *Z, const int *src, int nb,
+                               int NumofR, int MAXSwp, Grid2 g, int *cycles, int *status,
+                               int *k2out)
+{
+    size_t shm = sizeof(int) * (g.ncols + 1);
+    hipLaunchKernelGGL(rt_fsm2d_wg<Ref64>, dim3(nb), dim3(256), shm, st, vp, U, TTd, TTu, p, Z,
+                       src, NumofR, MAXSwp, g, cycles, status, k2out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rt_fsm2d_f32(hipStream_t st, const double *vp, float *U, float *TTd,
+                               float *TTu, const double *Z, const int *src, int nb, int NumofR,
+                               int MAXSwp, Grid2 g, int *cycles, int *status, int *k2out)
+{
+    size_t shm = sizeof(int) * (g.ncols + 1);
+    hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb), dim3(256), shm, st, vp, U, TTd, TTu,
+                       (float *)nullptr, Z, src, NumofR, MAXSwp, g, cycles, status, k2out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tadj2d_f64(hipStream_t st, double *U, double *p, unsigned char *tag,
+                             const double *T, const double *T_obs, double Cr, const int *rec,
+                             const double *nxy, const long *ch0, const int *srnum,
+                             const int *src, int nb, int MAXSwp, Grid2 g, double *EE,
+                             int *cycles)
+{
+    hipLaunchKernelGGL(tadj2d_wg<Ref64>, dim3(nb), dim3(256), 0, st, U, p, tag, T, T_obs, Cr,
+                       rec, nxy, ch0, srnum, src, MAXSwp, g, EE, cycles);
+    return hipGetLastError();
+}
+
+hipError_t launch_tadj2d_f32(hipStream_t st, float *U, unsigned char *tag, const float *T,
+                             const double *T_obs, double Cr, const int *rec, const double *nxy,
+                             const long *ch0, const int *srnum, const int *src, int nb,
+                             int MAXSwp, Grid2 g, double *EE, int *cycles)
+{
+    hipLaunchKernelGGL(tadj2d_wg<Fast32>, dim3(nb), dim3(256), 0, st, U, (float *)nullptr, tag,
+                       T, T_obs, Cr, rec, nxy, ch0, srnum, src, MAXSwp, g, EE, cycles);
+    return hipGetLastError();
+}
+
+hipError_t launch_radj2d_f64(hipStream_t st, double *UR, double *U, double *p,
+                             unsigned char *tag, const double *Tu, const double *TR_obs,
+                             double Cr, const int *rec, const double *nxy, const long *ch0,
+                             const int *srnum, const int *src, int nb, int NumofR, int MAXSwp,
+                             Grid2 g, double *EE, int *cycles)
+{
+    hipLaunchKernelGGL(radj2d_wg<Ref64>, dim3(nb), dim3(256), 0, st, UR, U, p, tag, Tu, TR_obs,
+                       Cr, rec, nxy, ch0, srnum, src, NumofR, MAXSwp, g, EE, cycles);
+    return hipGetLastError();
+}
+
+hipError_t launch_radj2d_f32(hipStream_t st, float *UR, float *U, unsigned char *tag,
+                             const float *Tu, const double *TR_obs, double Cr, const int *rec,
+                             const double *nxy, const long *ch0, const int *srnum,
+                             const int *src, int nb, int NumofR, int MAXSwp, Grid2 g,
+                             double *EE, int *cycles)
+{
+    hipLaunchKernelGGL(radj2d_wg<Fast32>, dim3(nb), dim3(256), 0, st, UR, U, (float *)nullptr,
+                       tag, Tu, TR_obs, Cr, rec, nxy, ch0, srnum, src, NumofR, MAXSwp, g, EE,
+                       cycles);
+    return hipGetLastError();
+}
+
+hipError_t launch_init3d_f64(hipStream_t st, double *U, long N, long srcnode)
+{
+    hipLaunchKernelGGL(init3d_f64, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, U, N,
+                       srcnode);
+    return hipGetLastError();
+}
+
+// One ordering's worth of hyperplane launches (forward).
+hipError_t launch_fsm3d_ordering_f64(hipStream_t st, const double *vp, double *U, double *p,
+                                     int *flag, int si, int sj, int sk, int s1, int s2, int s3,
+                                     double dx, double dy, double dz, int nrows, int ncols,
+                                     int nz, int first, int last)
+{
+    const int nplanes = nz + nrows + ncols - 2;
+    for (int d = 0; d < nplanes; d++) {
+        int k0 = d - (nrows - 1) - (ncols - 1);
+        if (k0 < 0) k0 = 0;
+        int k1 = d < nz - 1 ? d : nz - 1;
+        int i0 = d - k1 - (ncols - 1);
+        if (i0 < 0) i0 = 0;
+        int i1 = d - k0 < nrows - 1 ? d - k0 : nrows - 1;
+        const int nk = k1 - k0 + 1, ni = i1 - i0 + 1;
+        const int nt = nk * ni;
+        hipLaunchKernelGGL(fsm3d_plane_f64, dim3((nt + 255) / 256), dim3(256), 0, st, vp, U, p,
+                           flag, si, sj, sk, s1, s2, s3, d, k0, nk, i0, ni, dx, dy, dz, nrows,
+                           ncols, nz, first, last);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj3d_bc_f64(hipStream_t st, double *U, unsigned char *tag, const double *T,
+                               const double *T_obs, const int *rec, const double *nxyz, long ch0,
+                               int nrec, int si, int sj, int sk, double dx, double dy, double dz,
+                               int nrows, int ncols, int nz, double *EE, int *flag)
+{
+    hipLaunchKernelGGL(adj3d_bc_f64, dim3(1), dim3(64), 0, st, U, tag, T, T_obs, rec, nxyz, ch0,
+                       nrec, si, sj, sk, dx, dy, dz, nrows, ncols, nz, EE, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj3d_ordering_f64(hipStream_t st, double *U, const double *T,
+                                     const unsigned char *tag, double *p, int *flag, int s1,
+                                     int s2, int s3, double dx, double dy, double dz, int nrows,
+                                     int ncols, int nz, int first, int last)
+{
+    const int nzi = nz - 2, nri = nrows - 2, nci = ncols - 2;
+    if (nzi <= 0 || nri <= 0 || nci <= 0) return hipSuccess;
+    const int nplanes = nzi + nri + nci - 2;
+    for (int d = 0; d < nplanes; d++) {
+        int k0 = d - (nri - 1) - (nci - 1);
+        if (k0 < 0) k0 = 0;
+        int k1 = d < nzi - 1 ? d : nzi - 1;
+        int i0 = d - k1 - (nci - 1);
+        if (i0 < 0) i0 = 0;
+        int i1 = d - k0 < nri - 1 ? d - k0 : nri - 1;
+        const int nk = k1 - k0 + 1, ni = i1 - i0 + 1;
+        const int nt = nk * ni;
+        hipLaunchKernelGGL(adj3d_plane_f64, dim3((nt + 255) / 256), dim3(256), 0, st, U, T, tag,
+                           p, flag, s1, s2, s3, d, k0, nk, i0, ni, dx, dy, dz, nrows, ncols, nz,
+                           first, last);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pfast

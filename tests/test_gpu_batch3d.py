@@ -1,0 +1,95 @@
+"""GPU parity of the batched FP32 3D path against the FP64 oracle.
+
+Tolerances (stated by the path's precision contract):
+  traveltime: max|T32 - T64| / max|T64| <= 1e-5 over reached nodes
+  gradient  : ||g32 - g64||_2 / ||g64||_2 <= 1e-3
+  misfit    : |E32 - E64| / E64 <= 1e-4
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+T_TOL = 1e-5
+G_TOL = 1e-3
+E_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import pfast_amd
+    c = pfast_amd.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_fwdadj(c, nsrc):
+    N = c["vp"].size
+    g = np.zeros(N)
+    E = 0.0
+    Ts = []
+    for s in range(nsrc):
+        T, _ = O.fsm3d(c["vp"], c["src"], s, c["dx"], c["dy"], c["dz"])
+        lam, EE, _ = O.adjoint_fsm3d(T, c["T_obs"], c["rec"], c["nxyz"], c["src"], c["srnum"], s,
+                                     c["dx"], c["dy"], c["dz"])
+        O.oracle().orc_grad_accum(g, np.ascontiguousarray(lam.ravel()),
+                                  np.ascontiguousarray(c["vp"].ravel()), N)
+        E += EE
+        Ts.append(T)
+    return Ts, g.reshape(c["vp"].shape), E
+
+
+@pytest.mark.parametrize("seed,shape,spacing,nsrc", [
+    (21, (9, 12, 10), (10.0, 10.0, 10.0), 5),
+    (22, (11, 9, 13), (12.5, 10.0, 8.0), 6),
+    (23, (17, 19, 23), (25.0, 25.0, 25.0), 9),
+    (24, (6, 33, 20), (10.0, 10.0, 10.0), 4),
+])
+def test_batch3d_vs_oracle(ctx, seed, shape, spacing, nsrc):
+    import pfast_amd
+    c = cases.case3d(seed, shape=shape, spacing=spacing, nsrc=nsrc, nrec_per=8)
+    m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+    sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+    sh.run(adjoint=True, MAXSwp=100)
+    g32, E32 = sh.gradient()
+    Ts, g64, E64 = oracle_fwdadj(c, nsrc)
+    worst = 0.0
+    for s in range(nsrc):
+        T32 = sh.field(s)
+        T64 = Ts[s]
+        reach = T64 < 1e6
+        assert np.array_equal(reach, T32 < 1e6)
+        err = np.abs(T32 - T64)[reach].max() / T64[reach].max()
+        worst = max(worst, err)
+    gerr = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    eerr = abs(E32 - E64) / E64
+    print(f"seed {seed} shape {shape} T err {worst:.3e} grad err {gerr:.3e} misfit err {eerr:.3e}"
+          f" cycles {sh.cycles().tolist()}", flush=True)
+    assert worst <= T_TOL
+    assert gerr <= G_TOL
+    assert eerr <= E_TOL
+    picks = sh.picks()
+    ref = np.concatenate([Ts[s].ravel()[(c["rec"][c["srnum"][:s].sum():c["srnum"][:s + 1].sum(), 2]
+                                          * shape[1] + c["rec"][c["srnum"][:s].sum():c["srnum"][:s + 1].sum(), 0])
+                                         * shape[2] + c["rec"][c["srnum"][:s].sum():c["srnum"][:s + 1].sum(), 1]]
+                          for s in range(nsrc)])
+    assert np.abs(picks - ref).max() <= T_TOL * max(T.max() for T in Ts)
+    sh.close()
+    m.close()
+
+
+def test_batch3d_plain_source_matches_oracle(ctx):
+    """The resident single-source FP64 form equals the oracle bit for bit."""
+    import pfast_amd
+    c = cases.case3d(31, shape=(8, 9, 10), spacing=(10.0, 10.0, 10.0), nsrc=3)
+    m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+    sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+    for s in range(3):
+        EE, cyc = sh.plain_source(s)
+        T, cf = O.fsm3d(c["vp"], c["src"], s, c["dx"], c["dy"], c["dz"])
+        _, Eo, ca = O.adjoint_fsm3d(T, c["T_obs"], c["rec"], c["nxyz"], c["src"], c["srnum"], s,
+                                    c["dx"], c["dy"], c["dz"])
+        assert EE == Eo and cyc[0] == cf and cyc[1] == ca
