@@ -153,6 +153,31 @@ int pfast_ctx_work(pfast_ctx *ctx, int kind, double *node_sweeps);
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable);
 int pfast_ctx_timing_reset(pfast_ctx *ctx);
 
+/* ---------------------------------------------------------------------------------
+ * Optimiser support (host arrays), used by the reference-style drivers.
+ * ------------------------------------------------------------------------------- */
+/* FFT smoothing of a gradient, Dj <- Re IFFT[FFT(Dj) / (1 + nux^2 kx^2 + nuy^2 ky^2
+ * + nuz^2 kz^2)], air nodes (vp = 330) zeroed, on the GPU (hipFFT); wavenumbers
+ * built exactly as the reference builds them (including its odd-n ordering).
+ * Replaces regu3d, src/T_padjoint3d.h:205, and regu2d, src/RT_padjoint2d.h:506. */
+int pfast_regu3d(double *Dj, const double *vp, double nux, double nuy, double nuz, double dx,
+                 double dy, double dz, int nrows, int ncols, int nz);
+int pfast_regu2d(double *Dj, const double *vp, double nux, double nuz, double dx, double dz,
+                 int nrows, int ncols);
+/* beta_HS src/T_padjoint3d.h:264, beta_CGDESCENT :276, gd2sk :289, zoom :306,
+ * cubicInterp :316, magnitudeOf :328, meanOf :335 (2D twins in
+ * src/RT_padjoint2d.h:557-646).  N = number of grid nodes. */
+double pfast_beta_hs(const double *gk1, const double *gk0, const double *dk, long N);
+double pfast_beta_cgdescent(const double *gk1, const double *gk0, const double *dk, long N);
+double pfast_gd2sk(const double *gk1, const double *gk0, const double *dk, double alpha1,
+                   double *rhok, double *sk, double *yk, int Iterk, int lm, long N);
+double pfast_zoom(double alpha0, double alpha1, double phi0, double phi1, double dphi0,
+                  double dphi1);
+double pfast_cubic_interp(double alpha0, double alpha1, double phi0, double phi1, double dphi0,
+                          double dphi1);
+double pfast_magnitude_of(const double *g, long N);
+double pfast_mean_of(const double *g, long N);
+
 #ifdef __cplusplus
 }
 #endif

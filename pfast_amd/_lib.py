@@ -62,6 +62,18 @@ _SIGS = {
     "pfast_shots3d_cycles": [vpp, ip],
     "pfast_shots3d_plain_source": [vpp, i, i, pdbl, vpp, vpp, vpp],
     "pfast_ctx_work": [vpp, i, pdbl],
+    "pfast_regu3d": [dp, dp, d, d, d, d, d, d, i, i, i],
+    "pfast_regu2d": [dp, dp, d, d, d, d, i, i],
+}
+# entry points returning double rather than a status code
+_DSIGS = {
+    "pfast_beta_hs": [dp, dp, dp, C.c_long],
+    "pfast_beta_cgdescent": [dp, dp, dp, C.c_long],
+    "pfast_gd2sk": [dp, dp, dp, d, dp, dp, dp, i, i, C.c_long],
+    "pfast_zoom": [d, d, d, d, d, d],
+    "pfast_cubic_interp": [d, d, d, d, d, d],
+    "pfast_magnitude_of": [dp, C.c_long],
+    "pfast_mean_of": [dp, C.c_long],
 }
 
 _lib = None
@@ -78,6 +90,10 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        for name, args in _DSIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_double
         L.pfast_last_error.argtypes = []
         L.pfast_last_error.restype = C.c_char_p
         _lib = L
@@ -85,7 +101,7 @@ def lib():
 
 
 def exported_symbols():
-    return list(_SIGS) + ["pfast_last_error"]
+    return list(_SIGS) + list(_DSIGS) + ["pfast_last_error"]
 
 
 def check(rc):

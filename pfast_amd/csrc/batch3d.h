@@ -34,7 +34,8 @@ hipError_t launch_grad_reduce(hipStream_t st, const float *Lm, const double *vp,
                               long N, int nchunk, int nsrc);
 hipError_t launch_gather_picks(hipStream_t st, const float *T, const long *rnode,
                                const int *rsrc, int nrec, long N, double *out);
-hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, double *out);
+hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int strip_sign,
+                               double *out);
 hipError_t launch_cell_time(hipStream_t st, const double *vp, float *f, long N, double h,
                             int uniform);
 }  // namespace pfast

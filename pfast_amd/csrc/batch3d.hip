@@ -119,103 +119,164 @@ __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
     return bits;
 }
 
+// Per-thread constants of a tile task shared by the forward and adjoint sweeps.
+// Offsets are 32-bit element offsets inside one chunk's field (the chunk base
+// pointer is uniform), advanced by one level per hyperplane.
+template <int TI, int TJ, int C>
+struct Lane {
+    int c, li, lj, pipj, me;
+    bool valid, hreal, huse;
+    int hl;                 // halo LDS position (if huse)
+    unsigned oI, oH;        // element offsets of interior / halo node at level 0
+    int stepC;              // signed element step per level
+    unsigned node;          // i*ncols+j of this column (tile origin when !valid)
+};
+
+template <int TI, int TJ, int C>
+__device__ __forceinline__ Lane<TI, TJ, C> make_lane(const SlabArgs &a, const TileCtx &t, int s1,
+                                                     int s2, int s3)
+{
+    using G = Geo<TI, TJ, C>;
+    Lane<TI, TJ, C> l;
+    const int tid = threadIdx.x;
+    l.c = tid % C;
+    l.lj = (tid / C) % TJ;
+    l.li = tid / (C * TJ);
+    l.valid = l.li < t.TIe && l.lj < t.TJe;
+    l.pipj = (s1 > 0 ? l.li : t.TIe - 1 - l.li) + (s2 > 0 ? l.lj : t.TJe - 1 - l.lj);
+    l.me = ((l.li + 1) * G::HJ + (l.lj + 1)) * C + l.c;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const unsigned pC = plane * C;
+    const unsigned org = (unsigned)t.i0 * a.ncols + t.j0;
+    l.node = l.valid ? (unsigned)(t.i0 + l.li) * a.ncols + (t.j0 + l.lj) : org;
+    const HaloSlot<TI, TJ, C> hs = halo_slot<TI, TJ, C>(tid, t, a.nrows, a.ncols);
+    l.huse = hs.use;
+    l.hreal = hs.use && hs.inside;
+    l.hl = hs.lds;
+    const unsigned hnode = l.hreal ? (unsigned)hs.gnode : org;
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
+    l.oI = k0 * pC + l.node * C + l.c;
+    l.oH = k0 * pC + hnode * C + hs.c;
+    l.stepC = s3 > 0 ? (int)pC : -(int)pC;
+    return l;
+}
+
+__device__ __forceinline__ int wrap_inc(int s, int W) { return s == W - 1 ? 0 : s + 1; }
+
 // ------------------------------------------------------------------- forward
 template <int TI, int TJ, int C, int L, bool UNI>
 __global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s1, int s2,
                                                         int s3)
 {
     using G = Geo<TI, TJ, C>;
-    constexpr int W = G::W, SLAB = G::SLAB;
-    __shared__ float ring[W * SLAB];
-    __shared__ float fring[W * TI * TJ];
+    constexpr int W = G::W, SLAB = G::SLAB, FS = TI * TJ, SLOT = SLAB + FS, HJC = G::HJ * C;
+    __shared__ float ring[W * SLOT];  // per level: halo-padded slab, then the cell times
 
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live) return;
+    const Lane<TI, TJ, C> l = make_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
-    const int c = tid % C, lj = (tid / C) % TJ, li = tid / (C * TJ);
-    const bool valid = li < t.TIe && lj < t.TJe;
-    const int pi = s1 > 0 ? li : t.TIe - 1 - li;
-    const int pj = s2 > 0 ? lj : t.TJe - 1 - lj;
     const int nz = a.nz;
-    const long plane = (long)a.nrows * a.ncols;
-    float *__restrict__ F = a.field + (size_t)t.chunk * plane * nz * C;
-    const long mynode = (long)(t.i0 + li) * a.ncols + (t.j0 + lj);
-    const int me = (((li + 1) * G::HJ) + (lj + 1)) * C + c;
-    const int src = a.srcnode[t.chunk * C + c];
-    const HaloSlot<TI, TJ, C> hs = halo_slot<TI, TJ, C>(tid, t, a.nrows, a.ncols);
-    const bool fown = tid < TI * TJ && (tid / TJ) < t.TIe && (tid % TJ) < t.TJe;
-    const long fnode = (long)(t.i0 + tid / TJ) * a.ncols + (t.j0 + tid % TJ);
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    float *__restrict__ Fc = a.field + (size_t)t.chunk * plane * nz * C;
     const float *__restrict__ slow = a.slow;
+    // cell times: threads 0..FS-1 stream one node each
+    const bool fown = tid < FS && (tid / TJ) < t.TIe && (tid % TJ) < t.TJe;
+    const unsigned org = (unsigned)t.i0 * a.ncols + t.j0;
+    const unsigned fnode = fown ? (unsigned)(t.i0 + tid / TJ) * a.ncols + (t.j0 + tid % TJ) : org;
+    const int stepF = s3 > 0 ? (int)plane : -(int)plane;
+    unsigned oF = (s3 > 0 ? 0u : (unsigned)(nz - 1)) * plane + fnode;
+    unsigned oI = l.oI, oH = l.oH, oS = l.oI;
+    const int fme = SLAB + l.li * TJ + l.lj;
+    // level (in sweep order) at which this thread's column holds its own source
+    int srclev = -1;
+    {
+        const int sn = a.srcnode[t.chunk * C + l.c];
+        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
+            const int sk = (int)((unsigned)sn / plane);
+            srclev = s3 > 0 ? sk : nz - 1 - sk;
+        }
+    }
+    auto put = [&](int slot, float vi, float vh, float vf) {
+        float *r = ring + slot * SLOT;
+        if (l.valid) r[l.me] = vi;
+        if (l.huse) r[l.hl] = l.hreal ? vh : PFAST_INF_F;
+        if (fown) r[SLAB + tid] = vf;
+    };
 
-    auto kof = [&](int lev) { return s3 > 0 ? lev : nz - 1 - lev; };
-    auto ld_int = [&](int lev) -> float {
-        return valid ? F[((size_t)kof(lev) * plane + mynode) * C + c] : 0.0f;
-    };
-    auto ld_halo = [&](int lev) -> float {
-        return (hs.use && hs.inside) ? F[((size_t)kof(lev) * plane + hs.gnode) * C + hs.c]
-                                     : PFAST_INF_F;
-    };
-    auto ld_f = [&](int lev) -> float { return fown ? slow[(size_t)kof(lev) * plane + fnode] : 0.0f; };
-    auto st_lds = [&](int lev, float vi, float vh, float vf) {
-        float *r = ring + (lev % W) * SLAB;
-        if (valid) r[me] = vi;
-        if (hs.use) r[hs.lds] = vh;
-        if (fown) fring[(lev % W) * TI * TJ + tid] = vf;
-    };
-
-    // prologue: level 0 into LDS, levels 1..L into registers
-    st_lds(0, ld_int(0), ld_halo(0), ld_f(0));
+    // prologue: INF sentinel for level -1 (slot W-1), level 0 into slot 0, levels
+    // 1..L into registers
+    if (l.valid) ring[(W - 1) * SLOT + l.me] = PFAST_INF_F;
+    put(0, Fc[oI], Fc[oH], slow[oF]);
     float rI[L], rH[L], rF[L];
 #pragma unroll
     for (int u = 0; u < L; u++) {
-        const int lev = u + 1;
-        rI[u] = lev < nz ? ld_int(lev) : 0.0f;
-        rH[u] = lev < nz ? ld_halo(lev) : 0.0f;
-        rF[u] = lev < nz ? ld_f(lev) : 0.0f;
+        rI[u] = rH[u] = rF[u] = 0.0f;
+        if (u + 1 < nz) {
+            oI += l.stepC;
+            oH += l.stepC;
+            oF += stepF;
+            rI[u] = Fc[oI];
+            rH[u] = Fc[oH];
+            rF[u] = slow[oF];
+        }
     }
 
     bool changed = false;
     unsigned dirty = 0;  // bit (level % 32): this thread changed that level
     const int pend = nz + t.span + 2;
+    int vs = l.pipj == 0 ? 0 : W - l.pipj;  // ring slot of this thread's level p - pipj
+    int ss = 1;                              // slot of level p+1
+    int sr = W - (t.span + 2);               // slot of the retired level p-span-2
     for (int pb = 0; pb < pend; pb += L) {
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
-            // (a) slab p+1 from registers into the ring; (b) prefetch slab p+1+L
-            if (p + 1 < nz) st_lds(p + 1, rI[u], rH[u], rF[u]);
+            // (a) slab p+1 into the ring (INF sentinel past the last level);
+            // (b) prefetch slab p+1+L into registers
+            if (p + 1 < nz) put(ss, rI[u], rH[u], rF[u]);
+            else if (p + 1 == nz && l.valid) ring[ss * SLOT + l.me] = PFAST_INF_F;
             if (p + 1 + L < nz) {
-                rI[u] = ld_int(p + 1 + L);
-                rH[u] = ld_halo(p + 1 + L);
-                rF[u] = ld_f(p + 1 + L);
+                oI += l.stepC;
+                oH += l.stepC;
+                oF += stepF;
+                rI[u] = Fc[oI];
+                rH[u] = Fc[oH];
+                rF[u] = slow[oF];
             }
             lds_barrier();
             // (d) update this thread's node at level pk
-            const int pk = p - pi - pj;
-            if (valid && pk >= 0 && pk < nz) {
-                const float *r0 = ring + (pk % W) * SLAB;
-                const float self = r0[me];
-                const float xa = fminf(r0[me - C], r0[me + C]);
-                const float ya = fminf(r0[me - G::HJ * C], r0[me + G::HJ * C]);
-                const float zm = pk > 0 ? ring[((pk - 1) % W) * SLAB + me] : PFAST_INF_F;
-                const float zp = pk + 1 < nz ? ring[((pk + 1) % W) * SLAB + me] : PFAST_INF_F;
-                const float za = fminf(zm, zp);
-                const float fv = fring[(pk % W) * TI * TJ + li * TJ + lj];
+            const int pk = p - l.pipj;
+            if (l.valid && pk >= 0 && pk < nz) {
+                float *r0 = ring + vs * SLOT;
+                const float self = r0[l.me];
+                const float xa = fminf(r0[l.me - C], r0[l.me + C]);
+                const float ya = fminf(r0[l.me - HJC], r0[l.me + HJC]);
+                const int vm = vs == 0 ? W - 1 : vs - 1, vp = wrap_inc(vs, W);
+                const float za = fminf(ring[vm * SLOT + l.me], ring[vp * SLOT + l.me]);
+                const float fv = r0[fme];
                 float q;
                 if (UNI) q = eik3d_f32_uniform(xa, ya, za, fv);
                 else q = eik3d_f32_general(xa, ya, za, fv, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
-                const long node = (long)kof(pk) * plane + mynode;
-                if (q < self && node != src) {
-                    ring[(pk % W) * SLAB + me] = q;
+                if (q < self && pk != srclev) {
+                    r0[l.me] = q;
                     changed = true;
                     dirty |= 1u << (pk & 31);
                 }
             }
-            // (e) retire level p - span - 2 (no thread reads it any more)
+            vs = wrap_inc(vs, W);
+            // (e) retire level p-span-2 (no thread reads it any more): write back
+            // the nodes this thread changed
             const int lr = p - t.span - 2;
-            if (valid && lr >= 0 && lr < nz && (dirty >> (lr & 31)) & 1u) {
-                F[((size_t)kof(lr) * plane + mynode) * C + c] = ring[(lr % W) * SLAB + me];
-                dirty &= ~(1u << (lr & 31));
+            if (lr >= 0 && lr < nz) {
+                if (l.valid && ((dirty >> (lr & 31)) & 1u)) {
+                    Fc[oS] = ring[sr * SLOT + l.me];
+                    dirty &= ~(1u << (lr & 31));
+                }
+                oS += l.stepC;
             }
+            sr = wrap_inc(sr, W);
+            ss = wrap_inc(ss, W);
         }
     }
     const unsigned long long m = __ballot(changed);
@@ -225,85 +286,90 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s
 // ------------------------------------------------------------------- adjoint
 // lambda in a.field, traveltimes in a.tfield (read-only; a negative value marks a
 // receiver node of that source, whose lambda is fixed by the boundary condition).
+// Only interior nodes are updated, so no edge sentinels are needed.
 template <int TI, int TJ, int C, int L>
 __global__ __launch_bounds__(TI *TJ *C) void adj3d_slab(SlabArgs a, int D, int s1, int s2,
                                                         int s3)
 {
     using G = Geo<TI, TJ, C>;
-    constexpr int W = G::W, SLAB = G::SLAB;
+    constexpr int W = G::W, SLAB = G::SLAB, HJC = G::HJ * C;
     __shared__ float lring[W * SLAB];
     __shared__ float tring[W * SLAB];
 
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live) return;
-    const int tid = threadIdx.x;
-    const int c = tid % C, lj = (tid / C) % TJ, li = tid / (C * TJ);
-    const bool valid = li < t.TIe && lj < t.TJe;
-    const int pi = s1 > 0 ? li : t.TIe - 1 - li;
-    const int pj = s2 > 0 ? lj : t.TJe - 1 - lj;
+    const Lane<TI, TJ, C> l = make_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int nz = a.nz;
-    const long plane = (long)a.nrows * a.ncols;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
     const size_t cb = (size_t)t.chunk * plane * nz * C;
     float *__restrict__ Lm = a.field + cb;
     const float *__restrict__ Tm = a.tfield + cb;
-    const int gi = t.i0 + li, gj = t.j0 + lj;
-    const long mynode = (long)gi * a.ncols + gj;
-    const bool col_interior = gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
-    const int me = (((li + 1) * G::HJ) + (lj + 1)) * C + c;
-    const HaloSlot<TI, TJ, C> hs = halo_slot<TI, TJ, C>(tid, t, a.nrows, a.ncols);
-
-    auto kof = [&](int lev) { return s3 > 0 ? lev : nz - 1 - lev; };
-    auto gaddr = [&](int lev, long node, int cc) { return ((size_t)kof(lev) * plane + node) * C + cc; };
-    auto st_lds = [&](int lev, float li_, float lh, float ti_, float th) {
-        const int o = (lev % W) * SLAB;
-        if (valid) { lring[o + me] = li_; tring[o + me] = ti_; }
-        if (hs.use) { lring[o + hs.lds] = lh; tring[o + hs.lds] = th; }
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    const bool col_interior =
+        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
+    unsigned oI = l.oI, oH = l.oH, oS = l.oI;
+    auto put = [&](int slot, float vl, float vlh, float vt, float vth) {
+        const int o = slot * SLAB;
+        if (l.valid) {
+            lring[o + l.me] = vl;
+            tring[o + l.me] = vt;
+        }
+        if (l.huse) {
+            lring[o + l.hl] = vlh;
+            tring[o + l.hl] = l.hreal ? vth : PFAST_INF_F;
+        }
     };
-    // Halo nodes outside the domain are never read by an update (only interior
-    // nodes are updated), so their value is irrelevant.
-    auto ldLi = [&](int lev) { return valid ? Lm[gaddr(lev, mynode, c)] : 0.0f; };
-    auto ldTi = [&](int lev) { return valid ? Tm[gaddr(lev, mynode, c)] : 0.0f; };
-    auto ldLh = [&](int lev) { return (hs.use && hs.inside) ? Lm[gaddr(lev, hs.gnode, hs.c)] : 0.0f; };
-    auto ldTh = [&](int lev) { return (hs.use && hs.inside) ? Tm[gaddr(lev, hs.gnode, hs.c)] : PFAST_INF_F; };
-
-    st_lds(0, ldLi(0), ldLh(0), ldTi(0), ldTh(0));
+    put(0, Lm[oI], Lm[oH], Tm[oI], Tm[oH]);
     float rL[L], rLH[L], rT[L], rTH[L];
 #pragma unroll
     for (int u = 0; u < L; u++) {
-        const int lev = u + 1;
-        const bool ok = lev < nz;
-        rL[u] = ok ? ldLi(lev) : 0.0f;
-        rLH[u] = ok ? ldLh(lev) : 0.0f;
-        rT[u] = ok ? ldTi(lev) : 0.0f;
-        rTH[u] = ok ? ldTh(lev) : 0.0f;
+        rL[u] = rLH[u] = rT[u] = rTH[u] = 0.0f;
+        if (u + 1 < nz) {
+            oI += l.stepC;
+            oH += l.stepC;
+            rL[u] = Lm[oI];
+            rLH[u] = Lm[oH];
+            rT[u] = Tm[oI];
+            rTH[u] = Tm[oH];
+        }
     }
     const float wx = a.wx, wy = a.wy, wz = a.wz;
     bool changed = false;
     unsigned dirty = 0;
     const int pend = nz + t.span + 2;
+    int vs = l.pipj == 0 ? 0 : W - l.pipj;
+    int ss = 1;
+    int sr = W - (t.span + 2);
     for (int pb = 0; pb < pend; pb += L) {
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
-            if (p + 1 < nz) st_lds(p + 1, rL[u], rLH[u], rT[u], rTH[u]);
+            if (p + 1 < nz) put(ss, rL[u], rLH[u], rT[u], rTH[u]);
             if (p + 1 + L < nz) {
-                rL[u] = ldLi(p + 1 + L);
-                rLH[u] = ldLh(p + 1 + L);
-                rT[u] = ldTi(p + 1 + L);
-                rTH[u] = ldTh(p + 1 + L);
+                oI += l.stepC;
+                oH += l.stepC;
+                rL[u] = Lm[oI];
+                rLH[u] = Lm[oH];
+                rT[u] = Tm[oI];
+                rTH[u] = Tm[oH];
             }
             lds_barrier();
-            const int pk = p - pi - pj;
-            const int gk = kof(pk);
-            if (valid && col_interior && pk >= 0 && pk < nz && gk >= 1 && gk <= nz - 2) {
-                const int o = (pk % W) * SLAB + me;
-                const float tcr = tring[o];
-                if (!signbit(tcr)) {
-                    const int om = ((pk - 1) % W) * SLAB + me;  // pk >= 1 here
-                    const int op = ((pk + 1) % W) * SLAB + me;
-                    const float tc = tcr;
+            const int pk = p - l.pipj;
+            if (col_interior && pk >= 1 && pk <= nz - 2) {
+                const int o = vs * SLAB + l.me;
+                const float tc = tring[o];
+                if (!signbit(tc)) {
+                    // ring levels pk-1 / pk+1 are depths k-s3 / k+s3: name them by
+                    // absolute depth so that every ordering sums the terms in the
+                    // same order (another order rounds differently and makes a node
+                    // flip by an ulp from one ordering to the next, so the sweeps
+                    // would never become stationary)
+                    const int lo_ = (vs == 0 ? W - 1 : vs - 1) * SLAB + l.me;
+                    const int hi_ = wrap_inc(vs, W) * SLAB + l.me;
+                    const int om = s3 > 0 ? lo_ : hi_;  // depth k-1
+                    const int op = s3 > 0 ? hi_ : lo_;  // depth k+1
                     const float txm = fabsf(tring[o - C]), txp = fabsf(tring[o + C]);
-                    const float tym = fabsf(tring[o - G::HJ * C]), typ = fabsf(tring[o + G::HJ * C]);
+                    const float tym = fabsf(tring[o - HJC]), typ = fabsf(tring[o + HJC]);
                     const float tzm = fabsf(tring[om]), tzp = fabsf(tring[op]);
                     float nw = 0.0f;
                     const float I = PFAST_INF_F;
@@ -312,25 +378,30 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_slab(SlabArgs a, int D, int s
                         float den = 0.0f;
                         float num = adj_term(tc, txp, lring[o + C], wx, den);
                         num += adj_term(tc, txm, lring[o - C], wx, den);
-                        num += adj_term(tc, typ, lring[o + G::HJ * C], wy, den);
-                        num += adj_term(tc, tym, lring[o - G::HJ * C], wy, den);
+                        num += adj_term(tc, typ, lring[o + HJC], wy, den);
+                        num += adj_term(tc, tym, lring[o - HJC], wy, den);
                         num += adj_term(tc, tzp, lring[op], wz, den);
                         num += adj_term(tc, tzm, lring[om], wz, den);
-                        nw = fabsf(den) > 1e-9f ? num / den : 0.0f;
+                        nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
                     }
-                    const float old = lring[o];
-                    if (nw != old) {
+                    if (nw != lring[o]) {
                         lring[o] = nw;
                         changed = true;
                         dirty |= 1u << (pk & 31);
                     }
                 }
             }
+            vs = wrap_inc(vs, W);
             const int lr = p - t.span - 2;
-            if (valid && lr >= 0 && lr < nz && (dirty >> (lr & 31)) & 1u) {
-                Lm[gaddr(lr, mynode, c)] = lring[(lr % W) * SLAB + me];
-                dirty &= ~(1u << (lr & 31));
+            if (lr >= 0 && lr < nz) {
+                if (l.valid && ((dirty >> (lr & 31)) & 1u)) {
+                    Lm[oS] = lring[sr * SLAB + l.me];
+                    dirty &= ~(1u << (lr & 31));
+                }
+                oS += l.stepC;
             }
+            sr = wrap_inc(sr, W);
+            ss = wrap_inc(ss, W);
         }
     }
     const unsigned long long m = __ballot(changed);
@@ -462,12 +533,15 @@ __global__ void gather_picks(const float *__restrict__ T, const long *__restrict
     out[e] = (double)fabsf(T[((size_t)(s / C) * N + rnode[e]) * C + (s % C)]);
 }
 
-__global__ void gather_field(const float *__restrict__ F, int s, long N, int C,
+// Widen one source's field to FP64.  Traveltimes (strip_sign) may carry the
+// receiver marks of an adjoint in progress; adjoint fields are signed.
+__global__ void gather_field(const float *__restrict__ F, int s, long N, int C, int strip_sign,
                              double *__restrict__ out)
 {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    out[n] = (double)fabsf(F[((size_t)(s / C) * N + n) * C + (s % C)]);
+    const float v = F[((size_t)(s / C) * N + n) * C + (s % C)];
+    out[n] = (double)(strip_sign ? fabsf(v) : v);
 }
 
 __global__ void cell_time(const double *__restrict__ vp, float *__restrict__ f, long N,
@@ -560,10 +634,11 @@ hipError_t launch_gather_picks(hipStream_t st, const float *T, const long *rnode
     return hipGetLastError();
 }
 
-hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, double *out)
+hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int strip_sign,
+                               double *out)
 {
     hipLaunchKernelGGL(gather_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, s, N,
-                       PFAST_CHUNK, out);
+                       PFAST_CHUNK, strip_sign, out);
     return hipGetLastError();
 }
 

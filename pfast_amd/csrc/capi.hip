@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -915,7 +917,7 @@ int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field)
     pfast_model3d *m = s->m;
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols * m->nz;
-    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, s->fieldbuf));
+    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, which == 0, s->fieldbuf));
     HIPCHK(hipMemcpyAsync(field, s->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;

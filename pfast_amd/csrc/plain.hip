@@ -164,7 +164,7 @@ struct Fast32 {
         num += adj_term(tc, txm, lxm, (float)g.wx, den);
         num += adj_term(tc, tzp, lzp, (float)g.wz, den);
         num += adj_term(tc, tzm, lzm, (float)g.wz, den);
-        return fabsf(den) > 1e-9f ? num / den : 0.0f;
+        return fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
     }
     __device__ static real slow(const double *vp, long n) { return (float)(1.0 / vp[n]); }
 };

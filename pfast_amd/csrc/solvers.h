@@ -106,7 +106,10 @@ __device__ __forceinline__ double adj2d_ref(double tc, double txp, double txm, d
 }
 
 // ------------------------------------------------------------------ FP32 forms
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+// Hardware v_sqrt_f32 / v_rcp_f32 (about 1 ulp): the FP32 sweeps are checked
+// against the FP64 oracle at 1e-5, far above this rounding.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Uniform spacing h: f = h/v (travel time across one cell).  Neighbour minima
 // a, b, c in any order.  Same case structure as src/T_pfsm3d.h:106-119.
@@ -138,13 +141,13 @@ __device__ __forceinline__ float eik3d_f32_general(float a, float b, float c, fl
     const float d = b - c;
     const float W2 = wb + wc;
     const float disc2 = fmaf(W2, s * s, -wb * wc * d * d);
-    const float q2 = b + (fsqrt(disc2) - wc * d) / W2;
+    const float q2 = b + (fsqrt(disc2) - wc * d) * frcp(W2);
     const float db = a - b, dc = a - c;
     const float W3 = wa + wb + wc;
     const float P = fmaf(wb, db, wc * dc);
     const float Q = fmaf(wb * db, db, wc * dc * dc);
     const float disc3 = fmaf(P, P, -W3 * (Q - s * s));
-    const float q3 = disc3 >= 0.0f ? a + (fsqrt(disc3) - P) / W3 : PFAST_INF_F;
+    const float q3 = disc3 >= 0.0f ? a + (fsqrt(disc3) - P) * frcp(W3) : PFAST_INF_F;
     return q1 <= b ? q1 : (q2 <= a ? q2 : q3);
 }
 
@@ -160,7 +163,7 @@ __device__ __forceinline__ float eik2d_f32(float a, float b, float s, float dx, 
     const float d = hi - lo;
     const float W = wx + wz;
     const float disc = fmaf(W, s * s, -wx * wz * d * d);
-    const float q2 = disc >= 0.0f ? hi + (fsqrt(disc) - wlo * d) / W : PFAST_INF_F;
+    const float q2 = disc >= 0.0f ? hi + (fsqrt(disc) - wlo * d) * frcp(W) : PFAST_INF_F;
     return q1 <= hi ? q1 : q2;
 }
 
