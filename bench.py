@@ -105,6 +105,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = one per thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="batched sweep kernel variant (pfast_amd.sweep_variants())")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device-to-device copy of MIB MiB (3 times), the "
+                         "known-size kernel that scales the HBM byte counters of a profile")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -116,6 +121,18 @@ def main():
     import pfast_amd
     from pfast_amd import synth
 
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device copy "
+            f"({n * 4} B read + {n * 4} B written per launch)")
+        del a, b
+
     t0 = time.perf_counter()
     cfg = synth.config3d(nsrc=args.nsrc, nrec=args.nrec)
     vp = cfg["vp"]
@@ -124,6 +141,9 @@ def main():
     log(f"workload generated in {time.perf_counter() - t0:.1f}s: grid {vp.shape}, "
         f"{args.nsrc} sources x {args.nrec} receivers")
     ctx = pfast_amd.Context(0)
+    ctx.set_sweep(args.sweep)
+    sweep_name = pfast_amd.sweep_variants()[args.sweep]
+    log(f"sweep kernels: {sweep_name}")
     model = pfast_amd.Model3D(ctx, vp, h, h, h)
     shots = pfast_amd.Shots3D(model, cfg["src"], cfg["srnum"], cfg["rec"], cfg["nxyz"], cfg["T_obs"])
     ctx.sync()
@@ -160,7 +180,8 @@ def main():
     adj_bytes = adj_units * 12.0             # lambda read+write, T read
     fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
     adj_gbs = adj_bytes / (adj_ms * 1e-3) / 1e9
-    dom = "adj3d_slab" if adj_ms >= fwd_ms else "fwd3d_slab"
+    kfwd, kadj = ("fwd3d_slab", "adj3d_slab") if "slab" in sweep_name else ("fwd3d_col", "adj3d_col")
+    dom = kadj if adj_ms >= fwd_ms else kfwd
     log(f"kernels: fwd {fwd_ms:.1f} ms / {fwd_n} launches -> {fwd_gbs:.0f} GB/s; "
         f"adj {adj_ms:.1f} ms / {adj_n} launches -> {adj_gbs:.0f} GB/s")
 
@@ -185,7 +206,7 @@ def main():
     Ia = float(np.mean([p[1][1] for p in plain])) if plain else None
 
     # roofline of the dominant kernel (per-launch algorithmic bytes / mean launch time)
-    if dom == "adj3d_slab":
+    if dom == kadj:
         achieved, launches, avg_ms, per_launch = adj_gbs, adj_n, adj_ms / adj_n, adj_bytes / adj_n
     else:
         achieved, launches, avg_ms, per_launch = fwd_gbs, fwd_n, fwd_ms / fwd_n, fwd_bytes / fwd_n
@@ -201,12 +222,10 @@ def main():
         "config": {"workload": "3D diving-wave tomography, 201x201x101 nodes, h=25 m, "
                                f"{args.nsrc} sources x {args.nrec} receivers, forward+adjoint+gradient",
                    "nsrc": args.nsrc, "nrec": args.nrec, "grid": [101, 201, 201], "h": h,
-                   "parallelism": "all sources batched on 1 GPU"},
+                   "parallelism": "all sources batched on 1 GPU", "sweep": sweep_name},
         "roofline": roofline,
-        "kernels": {"fwd3d_slab": {"ms": round(fwd_ms, 2), "launches": fwd_n,
-                                   "GBps": round(fwd_gbs, 1)},
-                    "adj3d_slab": {"ms": round(adj_ms, 2), "launches": adj_n,
-                                   "GBps": round(adj_gbs, 1)}},
+        "kernels": {kfwd: {"ms": round(fwd_ms, 2), "launches": fwd_n, "GBps": round(fwd_gbs, 1)},
+                    kadj: {"ms": round(adj_ms, 2), "launches": adj_n, "GBps": round(adj_gbs, 1)}},
         "fp32_cycles_mean": [round(float(cyc[:, 0].mean()), 3), round(float(cyc[:, 1].mean()), 3)],
         "misfit": misfit,
     }

@@ -17,6 +17,7 @@ Arrays use the reference layouts: 3D fields are numpy arrays of shape
 (3D: row i, col j, depth k; 2D: row i, col j).  There is no CPU fallback.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -25,7 +26,7 @@ from ._lib import (PFAST_FWD_ADJ, PFAST_FWD_ONLY, FatalError, PfastError, check,
 
 INF = 9999999.0
 
-__all__ = ["fsm3d", "adjoint_fsm3d", "RT_fsm2d", "Tadjoint_fsm2d", "Radjoint_fsm2d",
+__all__ = ["fsm3d", "adjoint_fsm3d", "RT_fsm2d", "Tadjoint_fsm2d", "Radjoint_fsm2d", "sweep_variants",
            "Context", "Model3D", "Shots3D", "device_count", "PfastError", "FatalError", "INF"]
 
 
@@ -136,10 +137,13 @@ class Context:
         h = C.c_void_p()
         check(lib().pfast_ctx_create(int(device), C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # live models (closed before the context)
 
     def close(self):
         if self.h:
-            lib().pfast_ctx_destroy(self.h)
+            for m in list(self._children):
+                m.close()
+            check(lib().pfast_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -155,12 +159,26 @@ class Context:
         check(lib().pfast_ctx_timing_enable(self.h, 1 if enable else 0))
         check(lib().pfast_ctx_timing_reset(self.h))
 
+    def set_sweep(self, variant):
+        """Select the batched sweep kernel variant (0 = default); see sweep_variants()."""
+        check(lib().pfast_ctx_set_sweep(self.h, int(variant)))
+
     def timing_read(self, kind):
         """(device ms, launches, node-sweeps) of sweep kind 0 (forward) / 1 (adjoint)."""
         ms, n, w = C.c_double(0), C.c_long(0), C.c_double(0)
         check(lib().pfast_ctx_timing(self.h, int(kind), C.byref(ms), C.byref(n)))
         check(lib().pfast_ctx_work(self.h, int(kind), C.byref(w)))
         return ms.value, n.value, w.value
+
+
+def sweep_variants():
+    """Names of the batched sweep kernel variants, index = variant number."""
+    out = []
+    while True:
+        n = lib().pfast_sweep_name(len(out))
+        if n is None:
+            return out
+        out.append(n.decode())
 
 
 class Model3D:
@@ -176,6 +194,8 @@ class Model3D:
         check(lib().pfast_model3d_create(ctx.h, vp.ravel(), nrows, ncols, nz, self.dx, self.dy,
                                          self.dz, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # live shot sets (closed before the model)
+        ctx._children.add(self)
 
     def set_vp(self, vp):
         vp = _f64(vp)
@@ -184,7 +204,9 @@ class Model3D:
 
     def close(self):
         if self.h:
-            lib().pfast_model3d_destroy(self.h)
+            for s in list(self._children):
+                s.close()
+            check(lib().pfast_model3d_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -215,6 +237,7 @@ class Shots3D:
                                          ptr(self._rec), ptr(self._nxyz), ptr(self._tobs),
                                          C.byref(h)))
         self.h = h
+        model._children.add(self)
 
     def run(self, adjoint=True, MAXSwp=100):
         check(lib().pfast_shots3d_run(self.h, PFAST_FWD_ADJ if adjoint else PFAST_FWD_ONLY,

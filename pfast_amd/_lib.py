@@ -50,6 +50,7 @@ _SIGS = {
     "pfast_ctx_timing": [vpp, i, pdbl, plong],
     "pfast_ctx_timing_enable": [vpp, i],
     "pfast_ctx_timing_reset": [vpp],
+    "pfast_ctx_set_sweep": [vpp, i],
     "pfast_model3d_create": [vpp, dp, i, i, i, d, d, d, C.POINTER(vpp)],
     "pfast_model3d_set_vp": [vpp, dp],
     "pfast_model3d_destroy": [vpp],
@@ -96,12 +97,14 @@ def lib():
             fn.restype = C.c_double
         L.pfast_last_error.argtypes = []
         L.pfast_last_error.restype = C.c_char_p
+        L.pfast_sweep_name.argtypes = [i]
+        L.pfast_sweep_name.restype = C.c_char_p
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return list(_SIGS) + list(_DSIGS) + ["pfast_last_error"]
+    return list(_SIGS) + list(_DSIGS) + ["pfast_last_error", "pfast_sweep_name"]
 
 
 def check(rc):

@@ -17,11 +17,18 @@ struct SlabArgs {
     float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
 };
 
+// A sweep kernel variant: schedule (0 column window, 1 slab ring), tile rows x
+// columns, batch (column window) or prefetch (slab ring) depth.
+struct SweepVariant {
+    int kind, ti, tj, depth;
+    const char *name;
+};
+
 namespace pfast {
-hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D,
-                          int s1, int s2, int s3, int ntiles_diag);
-int tile_ti();
-int tile_tj();
+int sweep_variant_count();
+const SweepVariant &sweep_variant(int v);
+hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
+                          int uniform, int D, int s1, int s2, int s3, int ntiles_diag);
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
 hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,
                            int npick, long N, int unmark);

@@ -408,6 +408,351 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_slab(SlabArgs a, int D, int s
     if ((threadIdx.x & 63) == 0 && m) atomicOr(&a.changed[t.chunk], chunk_bits(m, C));
 }
 
+// ============================================================ column sweeps
+// Second schedule for the same sweep ("column window"): thread (li,lj,c) keeps its
+// own column of one source in registers -- levels m-1 (updated), m, m+1 (not yet
+// updated) of the level m = p - pi - pj it works on at step p -- so depth needs no
+// exchange at all.  Only the in-plane neighbours do: at every step each thread
+// publishes two values to LDS, its level m-1 (wanted by the neighbour one step
+// ahead of it, i.e. upstream in the sweep, which now works on level m) and its
+// level m+1 (wanted by the neighbour one step behind, downstream, which now works
+// on level m ... from its own point of view: the upstream neighbour of a thread on
+// level m is on level m+1 and holds the thread's level-m value as its level "m-1
+// updated"; the downstream neighbour is on level m-1 and holds it as "m+1 not yet
+// updated").  Tile-edge neighbours come from the neighbouring tiles in HBM, which
+// are either finished for this ordering (upstream tiles: earlier launches) or not
+// started (downstream: later launches).  The published array is double-buffered by
+// step parity, so one barrier per step suffices.  Memory traffic is batched: every
+// L steps one vmcnt(0) wait, then the stores of the previous batch's changed nodes
+// and the loads of the next batch, so every load has a whole batch to arrive.
+
+// Raw buffer access: 32-bit byte offsets from a uniform base, and an offset at or
+// past num_records (a level outside the column, where the unsigned offset wraps)
+// reads 0 without touching memory and never faults.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ float bload(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
+// vmcnt(0) (expcnt and lgkmcnt left at their maxima).  Issued through the builtin
+// so that the compiler's own wait insertion knows nothing is outstanding after it.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Per-thread constants of a column-window tile task.  Level lev (sweep order) of
+// the thread's column sits at byte offset oI + lev*step.
+template <int TI, int TJ, int C>
+struct ColLane {
+    int c, li, lj, pipj;
+    bool valid;
+    bool uxm, uxp, uym, uyp;   // that neighbour is a halo node (read from HBM)
+    int rxm, rxp, rym, ryp;    // LDS indices of the neighbours' published values
+    unsigned oI, oHX, oHY;     // byte offsets of level 0: own column, x halo, y halo
+    unsigned oF;               // byte offset of level 0 in the cell-time array
+    unsigned step, stepF;      // byte step per level (two's complement when s3 < 0)
+    unsigned node;
+};
+
+template <int TI, int TJ, int C>
+__device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SlabArgs &a, const TileCtx &t,
+                                                            int s1, int s2, int s3)
+{
+    constexpr int NT = TI * TJ * C, INFSLOT = 2 * NT;  // INF slot of each parity block
+    ColLane<TI, TJ, C> l;
+    const int tid = threadIdx.x;
+    l.c = tid % C;
+    l.lj = (tid / C) % TJ;
+    l.li = tid / (C * TJ);
+    l.valid = l.li < t.TIe && l.lj < t.TJe;
+    l.pipj = l.valid ? (s1 > 0 ? l.li : t.TIe - 1 - l.li) + (s2 > 0 ? l.lj : t.TJe - 1 - l.lj) : 0;
+    const bool exm = l.lj == 0, exp_ = l.lj == t.TJe - 1;
+    const bool eym = l.li == 0, eyp = l.li == t.TIe - 1;
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    // a tile-edge neighbour is a halo node inside the domain, or the domain edge
+    // (the INF slot); a one-column tile is the last of its row, so x+ is the edge
+    l.uxm = l.valid && exm && t.j0 > 0;
+    l.uxp = l.valid && !l.uxm && exp_ && t.j0 + t.TJe < a.ncols;
+    l.uym = l.valid && eym && t.i0 > 0;
+    l.uyp = l.valid && !l.uym && eyp && t.i0 + t.TIe < a.nrows;
+    // upstream neighbour (one step ahead): its "m-1" slot [0, NT); downstream: its
+    // "m+1" slot [NT, 2NT).  Idle threads read their own slot (unused).
+    const bool ok = l.valid;
+    l.rxm = !ok ? tid : exm ? INFSLOT : (s2 > 0 ? 0 : NT) + tid - C;
+    l.rxp = !ok ? tid : exp_ ? INFSLOT : (s2 > 0 ? NT : 0) + tid + C;
+    l.rym = !ok ? tid : eym ? INFSLOT : (s1 > 0 ? 0 : NT) + tid - C * TJ;
+    l.ryp = !ok ? tid : eyp ? INFSLOT : (s1 > 0 ? NT : 0) + tid + C * TJ;
+    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
+    l.node = l.valid ? (unsigned)gi * a.ncols + gj : 0u;
+    const int hxj = l.uxm ? t.j0 - 1 : t.j0 + t.TJe;
+    const int hyi = l.uym ? t.i0 - 1 : t.i0 + t.TIe;
+    const bool hx = l.uxm || l.uxp, hy = l.uym || l.uyp;
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
+    const unsigned NONE = 0x80000000u;  // out of range at every level (never loaded)
+    l.oI = 4u * (k0 * pC + l.node * C + l.c);
+    l.oHX = hx ? 4u * (k0 * pC + ((unsigned)gi * a.ncols + hxj) * C + l.c) : NONE;
+    l.oHY = hy ? 4u * (k0 * pC + ((unsigned)hyi * a.ncols + gj) * C + l.c) : NONE;
+    l.oF = 4u * (k0 * plane + l.node);
+    l.step = s3 > 0 ? 4u * pC : 0u - 4u * pC;
+    l.stepF = s3 > 0 ? 4u * plane : 0u - 4u * plane;
+    return l;
+}
+
+template <int TI, int TJ, int C, int L, bool UNI>
+__global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1, int s2, int s3)
+{
+    constexpr int NT = TI * TJ * C;
+    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
+    // per step parity: [m-1 | m+1][thread], then an INF slot (domain-edge neighbour)
+    constexpr int PB = 2 * NT + 1;
+    __shared__ float pub[2 * PB];
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const unsigned fbytes = 4u * plane * (unsigned)nz * C;
+    const rsrc_t RF = make_rsrc(a.field + (size_t)t.chunk * plane * nz * C, fbytes);
+    const rsrc_t RS = make_rsrc(a.slow, 4u * plane * (unsigned)nz);
+    const float INF = PFAST_INF_F;
+    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF;
+    // steps p at which this thread updates its node: pipj <= p < pipj + nz, except
+    // the level of its own source
+    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
+    int psrc = -1;
+    {
+        const int sn = a.srcnode[t.chunk * C + l.c];
+        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
+            const int sk = (int)((unsigned)sn / plane);
+            psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
+        }
+    }
+    const unsigned st = l.step, stF = l.stepF;
+    // queue of one batch: own column two levels ahead (m+2, INF outside the column),
+    // halos and cell time at level m itself
+    auto load_batch = [&](int pb, float *ow, float *hx, float *hy, float *fv) {
+        const int mb = pb - l.pipj;
+        const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const float o = bload(RF, l.oI + dm + (unsigned)(u + 2) * st);
+            ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF;
+            hx[u] = bload(RF, l.oHX + dm + (unsigned)u * st);
+            hy[u] = bload(RF, l.oHY + dm + (unsigned)u * st);
+            fv[u] = bload(RS, l.oF + dmf + (unsigned)u * stF);
+        }
+    };
+    // window at step 0 (level m0 = -pipj); level m0-1 is outside the column
+    const int m0 = -l.pipj;
+    float vprev = INF;
+    float vcur = l.valid && m0 == 0 ? bload(RF, l.oI) : INF;
+    float vnext = l.valid && m0 >= -1 ? bload(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
+    float qa[4][L], qb[4][L];
+    load_batch(0, qa[0], qa[1], qa[2], qa[3]);
+    __syncthreads();  // INF slot visible
+
+    bool changed = false;
+    float sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = 0.0f;
+        chg[u] = false;
+    }
+    // one batch of L steps from queue q (loaded a batch earlier); issues the loads
+    // of the following batch into nq first
+    auto batch = [&](int pb, float (*q)[L], float (*nq)[L]) {
+        wait_vm_all();
+        {  // changed nodes of the previous batch (steps pb-L .. pb-1)
+            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+            for (int u = 0; u < L; u++)
+                if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
+        }
+        load_batch(pb + L, nq[0], nq[1], nq[2], nq[3]);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int p = pb + u;
+            float *P = pub + (u & 1) * PB;
+            P[tid] = vprev;
+            P[NT + tid] = vnext;
+            lds_barrier();
+            const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
+            const float xa = fminf(l.uxm ? q[1][u] : pxm, l.uxp ? q[1][u] : pxp);
+            const float ya = fminf(l.uym ? q[2][u] : pym, l.uyp ? q[2][u] : pyp);
+            const float za = fminf(vprev, vnext);
+            float nv = vcur;
+            bool c = false;
+            if (p >= plo && p < phi && p != psrc) {
+                float qv;
+                if (UNI) qv = eik3d_f32_uniform(xa, ya, za, q[3][u]);
+                else qv = eik3d_f32_general(xa, ya, za, q[3][u], a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
+                c = qv < vcur;
+                nv = c ? qv : vcur;
+            }
+            chg[u] = c;
+            changed |= c;
+            sval[u] = nv;
+            vprev = nv;
+            vcur = vnext;
+            vnext = q[0][u];
+        }
+    };
+    const int pend = nz + t.span;
+    for (int pb = 0; pb < pend; pb += 2 * L) {
+        batch(pb, qa, qb);
+        batch(pb + L, qb, qa);
+    }
+    {
+        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
+        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
+    }
+    const unsigned long long bm = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+}
+
+// Adjoint, column window: lambda and traveltime travel together as float2 (the
+// traveltimes are read-only; a negative value marks a fixed receiver node).  Only
+// interior nodes are updated, and every neighbour they read is inside the domain.
+template <int TI, int TJ, int C, int L, bool S3POS>
+__global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1, int s2)
+{
+    constexpr int NT = TI * TJ * C;
+    constexpr int s3 = S3POS ? 1 : -1;
+    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
+    constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][thread], INF slot
+    __shared__ float2 pub[2 * PB];
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const size_t cb = (size_t)t.chunk * plane * nz * C;
+    const unsigned fbytes = 4u * plane * (unsigned)nz * C;
+    const rsrc_t RL = make_rsrc(a.field + cb, fbytes);
+    const rsrc_t RT = make_rsrc(a.tfield + cb, fbytes);
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    const bool col_interior =
+        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
+    // interior levels 1 .. nz-2 of interior columns
+    const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
+    const unsigned st = l.step;
+    const float wx = a.wx, wy = a.wy, wz = a.wz;
+    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
+
+    auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
+    auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
+        const unsigned dm = (unsigned)(pb - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            ow[u] = ld2(l.oI + dm + (unsigned)(u + 2) * st);
+            hx[u] = ld2(l.oHX + dm + (unsigned)u * st);
+            hy[u] = ld2(l.oHY + dm + (unsigned)u * st);
+        }
+    };
+    const int m0 = -l.pipj;
+    float2 vprev = make_float2(0.0f, 0.0f);
+    float2 vcur = l.valid && m0 == 0 ? ld2(l.oI) : make_float2(0.0f, 0.0f);
+    float2 vnext = l.valid && m0 >= -1 ? ld2(l.oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
+    float2 qa[3][L], qb[3][L];
+    load_batch(0, qa[0], qa[1], qa[2]);
+    __syncthreads();
+
+    bool changed = false;
+    float sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = 0.0f;
+        chg[u] = false;
+    }
+    auto batch = [&](int pb, float2 (*q)[L], float2 (*nq)[L]) {
+        wait_vm_all();
+        {
+            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+            for (int u = 0; u < L; u++)
+                if (chg[u]) bstore(RL, l.oI + d + (unsigned)u * st, sval[u]);
+        }
+        load_batch(pb + L, nq[0], nq[1], nq[2]);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int p = pb + u;
+            float2 *P = pub + (u & 1) * PB;
+            P[tid] = vprev;
+            P[NT + tid] = vnext;
+            lds_barrier();
+            const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
+            float nl = vcur.x;
+            bool c = false;
+            if (p >= plo && p < phi && !signbit(vcur.y)) {
+                const float2 xm = l.uxm ? q[1][u] : pxm;
+                const float2 xp = l.uxp ? q[1][u] : pxp;
+                const float2 ym = l.uym ? q[2][u] : pym;
+                const float2 yp = l.uyp ? q[2][u] : pyp;
+                // depth k-1 / k+1 by absolute depth (levels m-1 / m+1 are depths
+                // k-s3 / k+s3): every ordering sums the terms in one order, so a
+                // stationary node stays bitwise stationary
+                const float2 zm = S3POS ? vprev : vnext;
+                const float2 zp = S3POS ? vnext : vprev;
+                const float tc = vcur.y;
+                const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
+                            typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
+                float nw = 0.0f;
+                const float I = PFAST_INF_F;
+                if (!(tc >= I || txm >= I || txp >= I || tym >= I || typ >= I || tzm >= I ||
+                      tzp >= I)) {
+                    float den = 0.0f;
+                    float num = adj_term(tc, txp, xp.x, wx, den);
+                    num += adj_term(tc, txm, xm.x, wx, den);
+                    num += adj_term(tc, typ, yp.x, wy, den);
+                    num += adj_term(tc, tym, ym.x, wy, den);
+                    num += adj_term(tc, tzp, zp.x, wz, den);
+                    num += adj_term(tc, tzm, zm.x, wz, den);
+                    nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+                }
+                c = nw != vcur.x;
+                nl = nw;
+            }
+            chg[u] = c;
+            changed |= c;
+            sval[u] = nl;
+            vprev = make_float2(nl, vcur.y);
+            vcur = vnext;
+            vnext = q[0][u];
+        }
+    };
+    const int pend = nz + t.span;
+    for (int pb = 0; pb < pend; pb += 2 * L) {
+        batch(pb, qa, qb);
+        batch(pb + L, qb, qa);
+    }
+    {
+        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
+        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore(RL, l.oI + d + (unsigned)u * st, sval[u]);
+    }
+    const unsigned long long bm = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+}
+
 // ------------------------------------------------------------------- helpers
 // T = INF everywhere, 0 at each source's node (padding sources have node -1).
 template <int C>
@@ -556,27 +901,69 @@ __global__ void cell_time(const double *__restrict__ vp, float *__restrict__ f, 
 
 namespace pfast {
 
-#define PFAST_TILE_TI 8
-#define PFAST_TILE_TJ 8
-#define PFAST_PREFETCH 2
+// Sweep kernel variants (schedule, tile rows x columns, batch/prefetch depth).
+// Variant 0 is the default; the others stay selectable for measurement.
+static const SweepVariant kVariants[] = {
+    {0, 8, 8, 4, "column-window 8x8 L4"},
+    {1, 8, 8, 2, "slab-ring 8x8 L2"},
+    {0, 4, 4, 4, "column-window 4x4 L4"},
+    {0, 8, 4, 4, "column-window 8x4 L4"},
+    {0, 8, 8, 2, "column-window 8x8 L2"},
+    {0, 8, 8, 6, "column-window 8x8 L6"},
+};
 
-hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D,
-                          int s1, int s2, int s3, int ntiles_diag)
+int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+
+const SweepVariant &sweep_variant(int v) { return kVariants[v]; }
+
+namespace {
+template <int TI, int TJ, int L>
+void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
+                int s2, int s3, dim3 grid)
 {
-    constexpr int TI = PFAST_TILE_TI, TJ = PFAST_TILE_TJ, C = PFAST_CHUNK, L = PFAST_PREFETCH;
-    const dim3 grid((unsigned)(ntiles_diag * a.nact_p8)), block(TI * TJ * C);
-    if (adjoint)
-        hipLaunchKernelGGL((adj3d_slab<TI, TJ, C, L>), grid, block, 0, st, a, D, s1, s2, s3);
-    else if (uniform)
-        hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
-    else
-        hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2,
-                           s3);
+    constexpr int C = PFAST_CHUNK;
+    const dim3 block(TI * TJ * C);
+    if (adjoint) {
+        if (s3 > 0)
+            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2);
+        else
+            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2);
+    } else if (uniform) {
+        hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
+    } else {
+        hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2, s3);
+    }
+}
+}  // namespace
+
+hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
+                          int uniform, int D, int s1, int s2, int s3, int ntiles_diag)
+{
+    constexpr int C = PFAST_CHUNK;
+    const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));
+    switch (variant) {
+    case 0: launch_col<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 1: {
+        constexpr int TI = 8, TJ = 8, L = 2;
+        const dim3 block(TI * TJ * C);
+        if (adjoint)
+            hipLaunchKernelGGL((adj3d_slab<TI, TJ, C, L>), grid, block, 0, st, a, D, s1, s2, s3);
+        else if (uniform)
+            hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2,
+                               s3);
+        else
+            hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1,
+                               s2, s3);
+        break;
+    }
+    case 2: launch_col<4, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 3: launch_col<8, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 4: launch_col<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 5: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
-
-int tile_ti() { return PFAST_TILE_TI; }
-int tile_tj() { return PFAST_TILE_TJ; }
 
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk)
 {

@@ -29,6 +29,8 @@ struct pfast_ctx {
     double ms[2] = {0, 0};
     long launches[2] = {0, 0};
     double units[2] = {0, 0};  // node-sweeps of real sources issued (batched path)
+    int sweep = 0;             // batched sweep kernel variant
+    int nmodels = 0;           // live models bound to this context
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, int>> pending;  // (kind, event index of start)
     size_t evnext = 0;
@@ -163,6 +165,7 @@ int pfast_ctx_create(int device, pfast_ctx **out)
 int pfast_ctx_destroy(pfast_ctx *ctx)
 {
     if (!ctx) return PFAST_OK;
+    ARGCHK(ctx->nmodels == 0, "context still has live models: destroy them first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->cache)
@@ -180,6 +183,20 @@ int pfast_ctx_sync(pfast_ctx *ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     time_collect(ctx);
     return PFAST_OK;
+}
+
+int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ARGCHK(variant >= 0 && variant < pfast::sweep_variant_count(), "unknown sweep variant");
+    ctx->sweep = variant;
+    return PFAST_OK;
+}
+
+const char *pfast_sweep_name(int variant)
+{
+    if (variant < 0 || variant >= pfast::sweep_variant_count()) return nullptr;
+    return pfast::sweep_variant(variant).name;
 }
 
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable)
@@ -538,6 +555,7 @@ struct pfast_model3d {
     bool uniform;
     double *vp = nullptr;  // N doubles
     float *f = nullptr;    // N cell times / slowness
+    int nshots = 0;        // live shot sets bound to this model
 };
 
 struct pfast_shots3d {
@@ -600,7 +618,8 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
     pfast_model3d *m = s->m;
     pfast_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
-    const int TI = pfast::tile_ti(), TJ = pfast::tile_tj();
+    const SweepVariant &sv = pfast::sweep_variant(ctx->sweep);
+    const int TI = sv.ti, TJ = sv.tj;
     const int nti = (m->nrows + TI - 1) / TI, ntj = (m->ncols + TJ - 1) / TJ;
     SlabArgs a{};
     a.field = adjoint ? s->L : s->T;
@@ -638,7 +657,8 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
             for (int D = 0; D <= nti + ntj - 2; D++) {
                 const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
                 time_begin(ctx, adjoint ? 1 : 0);
-                HIPCHK(pfast::launch_sweep3d(st, a, adjoint, m->uniform, D, s1, s2, s3, hi - lo + 1));
+                HIPCHK(pfast::launch_sweep3d(st, a, ctx->sweep, adjoint, m->uniform, D, s1, s2, s3,
+                                              hi - lo + 1));
                 time_end(ctx);
             }
         }
@@ -674,6 +694,7 @@ int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     HIPCHK(hipSetDevice(ctx->device));
     auto *m = new pfast_model3d();
     m->ctx = ctx;
+    ctx->nmodels++;  // undone by pfast_model3d_destroy, also on the failure paths below
     m->nrows = nrows;
     m->ncols = ncols;
     m->nz = nz;
@@ -710,7 +731,9 @@ int pfast_model3d_set_vp(pfast_model3d *m, const double *vp)
 int pfast_model3d_destroy(pfast_model3d *m)
 {
     if (!m) return PFAST_OK;
+    ARGCHK(m->nshots == 0, "model still has live shot sets: destroy them first");
     (void)hipStreamSynchronize(m->ctx->stream);
+    m->ctx->nmodels--;
     if (m->vp) (void)hipFree(m->vp);
     if (m->f) (void)hipFree(m->f);
     delete m;
@@ -721,6 +744,7 @@ int pfast_shots3d_destroy(pfast_shots3d *s)
 {
     if (!s) return PFAST_OK;
     (void)hipStreamSynchronize(s->m->ctx->stream);
+    s->m->nshots--;
     void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
                     s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
                     s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active};
@@ -738,22 +762,24 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     ARGCHK(m && src && srnum && out && nsrc > 0, "bad argument");
     const int nrows = m->nrows, ncols = m->ncols, nz = m->nz;
     const long N = (long)nrows * ncols * nz;
+    // the sweep kernels address one chunk field (C sources) with 32-bit byte offsets
+    ARGCHK(N * PFAST_CHUNK * 4 <= (1L << 30), "grid too large for the batched solver (a field "
+                                              "of one source chunk must stay within 1 GiB)");
+    for (int i = 0; i < nsrc; i++) ARGCHK(srnum[i] >= 0, "negative srnum");
     auto *s = new pfast_shots3d();
     s->m = m;
+    m->nshots++;  // undone by pfast_shots3d_destroy, also on the failure paths below
     s->nsrc = nsrc;
     s->nchunk = (nsrc + PFAST_CHUNK - 1) / PFAST_CHUNK;
     s->src.assign(src, src + 3 * nsrc);
     s->srnum.assign(srnum, srnum + nsrc);
     s->ch0.assign(nsrc + 1, 0);
-    for (int i = 0; i < nsrc; i++) {
-        ARGCHK(srnum[i] >= 0, "negative srnum");
-        s->ch0[i + 1] = s->ch0[i] + srnum[i];
-    }
+    for (int i = 0; i < nsrc; i++) s->ch0[i + 1] = s->ch0[i] + srnum[i];
     s->nrec = s->ch0[nsrc];
     for (int i = 0; i < nsrc; i++) {
         const int *q = src + 3 * i;
         if (!(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols && q[2] >= 0 && q[2] < nz)) {
-            delete s;
+            pfast_shots3d_destroy(s);
             pfast::set_error("invalid argument: source node outside the grid");
             return PFAST_ERR_ARG;
         }
@@ -762,7 +788,7 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         for (long r = 0; r < s->nrec; r++) {
             const int *q = rec + 3 * r;
             if (!(q[0] >= 0 && q[0] < nrows && q[1] >= 0 && q[1] < ncols && q[2] >= 0 && q[2] < nz)) {
-                delete s;
+                pfast_shots3d_destroy(s);
                 pfast::set_error("invalid argument: receiver node outside the grid");
                 return PFAST_ERR_ARG;
             }

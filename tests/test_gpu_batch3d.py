@@ -47,6 +47,8 @@ def oracle_fwdadj(c, nsrc):
     (22, (11, 9, 13), (12.5, 10.0, 8.0), 6),
     (23, (17, 19, 23), (25.0, 25.0, 25.0), 9),
     (24, (6, 33, 20), (10.0, 10.0, 10.0), 4),
+    (25, (20, 27, 35), (25.0, 25.0, 25.0), 7),    # several tiles, ragged edges
+    (26, (13, 30, 17), (12.5, 10.0, 8.0), 5),     # anisotropic spacing, ragged edges
 ])
 def test_batch3d_vs_oracle(ctx, seed, shape, spacing, nsrc):
     import pfast_amd
@@ -93,3 +95,65 @@ def test_batch3d_plain_source_matches_oracle(ctx):
         _, Eo, ca = O.adjoint_fsm3d(T, c["T_obs"], c["rec"], c["nxyz"], c["src"], c["srnum"], s,
                                     c["dx"], c["dy"], c["dz"])
         assert EE == Eo and cyc[0] == cf and cyc[1] == ca
+
+
+@pytest.mark.parametrize("seed,shape,spacing", [
+    (41, (20, 27, 35), (25.0, 25.0, 25.0)),
+    (42, (13, 30, 17), (12.5, 10.0, 8.0)),
+    (43, (7, 9, 41), (10.0, 10.0, 10.0)),
+])
+def test_sweep_variants_bitwise_identical(seed, shape, spacing):
+    """Every sweep kernel variant (schedule, tile shape, batch depth) performs the
+    same per-node arithmetic in the same order, so traveltimes, adjoint fields,
+    cycle counts, gradient and misfit agree bit for bit."""
+    import pfast_amd
+    c = cases.case3d(seed, shape=shape, spacing=spacing, nsrc=7, nrec_per=10)
+    names = pfast_amd.sweep_variants()
+    assert len(names) >= 2
+    ref = None
+    for v, name in enumerate(names):
+        ctx = pfast_amd.Context(0)
+        ctx.set_sweep(v)
+        m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+        sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+        sh.run(adjoint=True, MAXSwp=100)
+        g, E = sh.gradient()
+        out = (np.stack([sh.field(s) for s in range(7)]),
+               np.stack([sh.field(s, adjoint=True) for s in range(7)]), sh.cycles(), g, E)
+        sh.close()
+        m.close()
+        ctx.close()
+        print(f"variant {v} ({name}): cycles {out[2].tolist()}", flush=True)
+        if ref is None:
+            ref = out
+            continue
+        assert np.array_equal(out[0], ref[0]), name
+        assert np.array_equal(out[1], ref[1]), name
+        assert np.array_equal(out[2], ref[2]), name
+        assert np.array_equal(out[3], ref[3]) and out[4] == ref[4], name
+
+
+def test_sweep_variant_rejects_unknown():
+    import pfast_amd
+    ctx = pfast_amd.Context(0)
+    with pytest.raises(pfast_amd.PfastError):
+        ctx.set_sweep(len(pfast_amd.sweep_variants()))
+    ctx.close()
+
+
+def test_parent_close_closes_children():
+    """A context or model is never freed under a live child: the C ABI refuses,
+    and the Python wrappers close children first (no dangling stream pointer when
+    a test fails before its own cleanup)."""
+    import pfast_amd
+    from pfast_amd._lib import PFAST_ERR_ARG, lib
+    c = cases.case3d(51, shape=(6, 7, 8), spacing=(10.0, 10.0, 10.0), nsrc=2)
+    ctx = pfast_amd.Context(0)
+    m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+    sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+    assert lib().pfast_ctx_destroy(ctx.h) == PFAST_ERR_ARG
+    assert lib().pfast_model3d_destroy(m.h) == PFAST_ERR_ARG
+    sh.run(adjoint=False)
+    ctx.close()
+    assert sh.h is None and m.h is None and ctx.h is None
+    sh.close()

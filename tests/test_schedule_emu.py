@@ -1,8 +1,10 @@
-"""CPU check of the batched 3D sweep schedule (pfast_amd/csrc/batch3d.hip).
+"""CPU check of the batched 3D sweep schedules (pfast_amd/csrc/batch3d.hip).
 
-tests/emu/slab_schedule.cpp replays the slab-ring kernels' schedule on the CPU
-(tile anti-diagonal launches, per-thread level skew, LDS ring slots, register
-prefetch, dirty-node write-back) and compares every one of the 8 orderings,
+tests/emu/sweep_schedule.cpp replays the kernels' schedules on the CPU -- the
+slab ring (tile anti-diagonal launches, per-thread level skew, LDS ring slots,
+register prefetch, dirty-node write-back) and the column window (register
+windows, parity-buffered exchange of levels m-1 / m+1, halo and INF-slot
+neighbours, batched loads and delayed stores) -- and compares every one of the 8 orderings,
 bit for bit, with the plain lexicographic sweep in the reference's loop order
 (src/T_pfsm3d.h:131-185 forward, src/T_padjoint3d.h:73-172 adjoint) using the
 same per-node arithmetic.  Partial edge tiles, nz not a multiple of the
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 HERE = Path(__file__).resolve().parent
-SRC = HERE / "emu" / "slab_schedule.cpp"
+SRC = HERE / "emu" / "sweep_schedule.cpp"
 INF = np.float32(9999999.0)
 
 
@@ -25,31 +27,35 @@ def emu(tmp_path_factory):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    exe = tmp_path_factory.mktemp("emu") / "slab_schedule"
+    exe = tmp_path_factory.mktemp("emu") / "sweep_schedule"
     subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", str(SRC), "-o", str(exe)],
                    check=True)
     return exe
 
 
-def _run(exe, tmp, mode, shape, tile, T, U0, src):
+def _run(exe, tmp, mode, shape, tile, T, U0, src, sched="slab"):
     nz, nr, nc = shape
-    f = tmp / f"in_{mode}_{nz}_{nr}_{nc}_{tile[0]}_{tile[1]}_{tile[2]}.bin"
+    f = tmp / f"in_{mode}_{sched}_{nz}_{nr}_{nc}_{tile[0]}_{tile[1]}_{tile[2]}.bin"
     with open(f, "wb") as fh:
         fh.write(np.ascontiguousarray(T, np.float32).tobytes())
         fh.write(np.ascontiguousarray(U0, np.float32).tobytes())
         fh.write(np.int32(src).tobytes())
-    r = subprocess.run([str(exe), mode, str(nr), str(nc), str(nz), *map(str, tile), str(f)],
+    r = subprocess.run([str(exe), mode, sched, str(nr), str(nc), str(nz), *map(str, tile), str(f)],
                        capture_output=True, text=True)
     return r.returncode, r.stdout
 
 
 SHAPES = [((11, 19, 13), (8, 8, 2)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 8, 3)),
           ((7, 5, 23), (8, 4, 1)), ((12, 16, 9), (8, 8, 4))]
+# column window: the batch depth L is even (static step parity inside a batch)
+COL_SHAPES = [((11, 19, 13), (8, 8, 4)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 4, 4)),
+              ((7, 5, 23), (8, 4, 6)), ((12, 16, 9), (8, 8, 4)), ((3, 17, 9), (8, 8, 4))]
+CASES = [(sh, t, "slab") for sh, t in SHAPES] + [(sh, t, "col") for sh, t in COL_SHAPES]
 
 
-@pytest.mark.parametrize("shape,tile", SHAPES)
+@pytest.mark.parametrize("shape,tile,sched", CASES)
 @pytest.mark.parametrize("where", ["interior", "face"])
-def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, where):
+def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched, where):
     nz, nr, nc = shape
     rng = np.random.default_rng(nz * 1000 + nr * 10 + nc)
     ftime = (25.0 / rng.uniform(1500, 6000, size=shape)).astype(np.float32)
@@ -57,13 +63,13 @@ def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, where
     sn = int(np.ravel_multi_index(src, shape))
     U0 = np.full(shape, INF, np.float32)
     U0[src] = 0.0
-    rc, out = _run(emu, tmp_path, "fwd", shape, tile, ftime, U0, sn)
+    rc, out = _run(emu, tmp_path, "fwd", shape, tile, ftime, U0, sn, sched)
     assert rc == 0, out
     assert "mismatches 0" in out
 
 
-@pytest.mark.parametrize("shape,tile", SHAPES)
-def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile):
+@pytest.mark.parametrize("shape,tile,sched", CASES)
+def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched):
     nz, nr, nc = shape
     rng = np.random.default_rng(7 + nz + nr + nc)
     k, i, j = np.meshgrid(np.arange(nz), np.arange(nr), np.arange(nc), indexing="ij")
@@ -78,16 +84,20 @@ def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile):
     rj = rng.integers(1, nc - 1, nrec)
     T[rk, ri, rj] = -T[rk, ri, rj]
     L0[rk, ri, rj] = rng.normal(0, 50, nrec).astype(np.float32)
-    rc, out = _run(emu, tmp_path, "adj", shape, tile, T, L0, -1)
+    rc, out = _run(emu, tmp_path, "adj", shape, tile, T, L0, -1, sched)
     assert rc == 0, out
     assert "mismatches 0" in out
 
 
-@pytest.mark.parametrize("plant,mode", [("PLANT_NO_SKEW", "fwd"), ("PLANT_RING_DEPTH", "adj")])
-def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode):
+@pytest.mark.parametrize("plant,mode,sched", [("PLANT_NO_SKEW", "fwd", "slab"),
+                                              ("PLANT_RING_DEPTH", "adj", "slab"),
+                                              ("PLANT_RING_DEPTH", "adj", "col"),
+                                              ("PLANT_SWAP_EXCHANGE", "fwd", "col")])
+def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode, sched):
     """The comparison is not vacuous: the emulator built with a deliberate
     schedule mistake (all columns on one level; z-neighbours named by ring order
-    instead of absolute depth) reports mismatches."""
+    instead of absolute depth; upstream and downstream exchange slots swapped)
+    reports mismatches."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
@@ -111,5 +121,7 @@ def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode):
         T[rk, ri, rj] = -T[rk, ri, rj]
         U0[rk, ri, rj] = rng.normal(0, 50, 30).astype(np.float32)
         src = -1
-    rc, out = _run(exe, tmp_path, mode, shape, tile, T, U0, src)
+    if sched == "col":
+        tile = (8, 8, 4)
+    rc, out = _run(exe, tmp_path, mode, shape, tile, T, U0, src, sched)
     assert rc == 1 and "mismatches 0" not in out, out
