@@ -17,8 +17,9 @@ struct SlabArgs {
     float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
 };
 
-// A sweep kernel variant: schedule (0 column window, 1 slab ring), tile rows x
-// columns, batch (column window) or prefetch (slab ring) depth.
+// A sweep kernel variant: schedule (0 column window, 1 slab ring, 2 loader wave),
+// tile rows x columns, batch (column window), prefetch (slab ring) or forward
+// lookahead (loader wave) depth.
 struct SweepVariant {
     int kind, ti, tj, depth;
     const char *name;

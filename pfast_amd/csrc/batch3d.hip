@@ -448,6 +448,7 @@ __device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
 // so that the compiler's own wait insertion knows nothing is outstanding after it.
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+
 // Per-thread constants of a column-window tile task.  Level lev (sweep order) of
 // the thread's column sits at byte offset oI + lev*step.
 template <int TI, int TJ, int C>
@@ -753,6 +754,333 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
 }
 
+// ============================================================ loader-wave sweeps
+// Third schedule ("loader wave"): the column-window sweep with its HBM reads moved
+// out of the compute waves.  One extra wave per block only issues loads: at step p
+// it streams the data every compute thread needs at step p + A (own column two
+// levels ahead, tile-edge halos, cell time) from HBM straight into a step-indexed
+// LDS ring (buffer -> LDS DMA; one dwordx4 lane carries the C = 4 sources of a
+// column), then waits until step p + 1's data has landed and joins the step
+// barrier.  Because that wave issues loads only, vmcnt counts them in order and
+// the wait is exact, so A steps of HBM latency are hidden.  The compute waves read
+// everything from LDS and store changed nodes as they go (never waiting on them).
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ void dma16(rsrc_t r, void *lds, unsigned off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t *)lds, 16, off, 0, 0, 0);
+}
+__device__ __forceinline__ void dma4(rsrc_t r, void *lds, unsigned off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t *)lds, 4, off, 0, 0, 0);
+}
+// s_waitcnt vmcnt(n), n <= 63 (expcnt / lgkmcnt at their maxima)
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
+
+// The consumer (compute thread) of a tile-edge halo entry h of a step slot, and
+// the halo node it needs.  Entries 0 .. 2TI-1: x halos (row h/2, side h%2: 0 = x-,
+// 1 = x+); then 2TJ y halos (column, side 0 = y-, 1 = y+).
+template <int TI, int TJ>
+__device__ __forceinline__ bool halo_entry(int h, const TileCtx &t, int nrows, int ncols, int &cli,
+                                           int &clj, unsigned &hnode)
+{
+    int gi, gj;
+    if (h < 2 * TI) {
+        cli = h / 2;
+        const int side = h % 2;
+        clj = side ? t.TJe - 1 : 0;
+        gi = t.i0 + cli;
+        gj = side ? t.j0 + t.TJe : t.j0 - 1;
+        if (cli >= t.TIe) return false;
+    } else {
+        clj = (h - 2 * TI) / 2;
+        const int side = (h - 2 * TI) % 2;
+        cli = side ? t.TIe - 1 : 0;
+        gi = side ? t.i0 + t.TIe : t.i0 - 1;
+        gj = t.j0 + clj;
+        if (clj >= t.TJe) return false;
+    }
+    if (gi < 0 || gi >= nrows || gj < 0 || gj >= ncols) return false;
+    hnode = (unsigned)gi * ncols + gj;
+    return true;
+}
+
+template <int TI, int TJ>
+__device__ __forceinline__ int pipj_of(int li, int lj, const TileCtx &t, int s1, int s2)
+{
+    return (s1 > 0 ? li : t.TIe - 1 - li) + (s2 > 0 ? lj : t.TJe - 1 - lj);
+}
+
+template <int TI, int TJ, int C, int A, bool UNI>
+__global__ __launch_bounds__(TI *TJ *C + 64) void fwd3d_dma(SlabArgs a, int D, int s1, int s2,
+                                                            int s3)
+{
+    static_assert(C == 4, "one dwordx4 lane per column");
+    constexpr int NT = TI * TJ * C, NCOL = TI * TJ, NH = 2 * (TI + TJ);
+    static_assert(NCOL <= 64 && NH <= 64, "one loader lane per column / halo node");
+    constexpr int R = A + 1;  // ring slots: step p + A reuses the slot of step p - 1
+    constexpr int S_HALO = NT, S_F = NT + NH * C, SLOT = S_F + NCOL;  // floats per slot
+    static_assert(SLOT % 4 == 0, "16-byte aligned slots");
+    constexpr int PB = 2 * NT + 1;
+    __shared__ float pub[2 * PB];
+    __shared__ __attribute__((aligned(16))) float ring[R * SLOT];
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
+    const rsrc_t RF = make_rsrc(a.field + (size_t)t.chunk * plane * nz * C, 4u * pC * nz);
+    const rsrc_t RS = make_rsrc(a.slow, 4u * plane * nz);
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
+    const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
+    const unsigned stF = s3 > 0 ? 4u * plane : 0u - 4u * plane;
+    const unsigned NONE = 0x80000000u;
+    const float INF = PFAST_INF_F;
+    const int pend = nz + t.span;
+    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF;
+    __syncthreads();
+
+    if (tid >= NT) {  // ------------------------------------------ loader wave
+        const int l = tid - NT;
+        // own column: level p - pipj + 2 for step p
+        unsigned oOwn = NONE, oF = NONE, oH = NONE;
+        if (l < NCOL) {
+            const int li = l / TJ, lj = l % TJ;
+            if (li < t.TIe && lj < t.TJe) {
+                const int pp = pipj_of<TI, TJ>(li, lj, t, s1, s2);
+                const unsigned node = (unsigned)(t.i0 + li) * a.ncols + t.j0 + lj;
+                oOwn = 4u * (k0 * pC + node * C) + (unsigned)(2 - pp) * st;
+                oF = 4u * (k0 * plane + node) + (unsigned)(-pp) * stF;
+            }
+        }
+        if (l < NH) {
+            int cli, clj;
+            unsigned hn;
+            if (halo_entry<TI, TJ>(l, t, a.nrows, a.ncols, cli, clj, hn))
+                oH = 4u * (k0 * pC + hn * C) + (unsigned)(-pipj_of<TI, TJ>(cli, clj, t, s1, s2)) * st;
+        }
+        // the data of step q into slot q % R; each DMA only on the lanes that own an
+        // entry (a lane writes its 16 / 4 bytes at lane * size from the base even
+        // when its load is out of range), lane 0 always among them, so every issue
+        // is exactly three VMEM instructions
+        auto issue = [&](int q) {
+            float *S = ring + (q % R) * SLOT;
+            if (l < NCOL) {
+                dma16(RF, S, oOwn);
+                dma4(RS, S + S_F, oF);
+            }
+            if (l < NH) dma16(RF, S + S_HALO, oH);
+            oOwn += st;
+            oH += st;
+            oF += stF;
+        };
+        for (int q = 0; q < A; q++) issue(q);
+        for (int p = 0; p < pend; p++) {
+            wait_vm<3 * (A - 1)>();  // step p's three loads have landed
+            __builtin_amdgcn_s_barrier();
+            issue(p + A);  // slot of step p - 1, whose readers passed this barrier
+        }
+        wait_vm<0>();
+        return;
+    }
+
+    // ------------------------------------------------------------ compute waves
+    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
+    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
+    int psrc = -1;
+    {
+        const int sn = a.srcnode[t.chunk * C + l.c];
+        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
+            const int sk = (int)((unsigned)sn / plane);
+            psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
+        }
+    }
+    // ring indices of this thread's entries
+    const int iOwn = tid;
+    const int col = l.li * TJ + l.lj;
+    const int iF = S_F + col;
+    const int iHX = S_HALO + (l.li * 2 + (l.uxp ? 1 : 0)) * C + l.c;
+    const int iHY = S_HALO + (2 * TI + l.lj * 2 + (l.uyp ? 1 : 0)) * C + l.c;
+    const int m0 = -l.pipj;
+    float vprev = INF;
+    float vcur = l.valid && m0 == 0 ? bload(RF, l.oI) : INF;
+    float vnext = l.valid && m0 >= -1 ? bload(RF, l.oI + (unsigned)(m0 + 1) * l.step) : INF;
+    unsigned oS = l.oI + (unsigned)m0 * l.step;  // byte offset of this thread's level at step p
+    bool changed = false;
+    int slot = 0;
+    for (int p = 0; p < pend; p++) {
+        float *P = pub + (p & 1) * PB;
+        P[tid] = vprev;
+        P[NT + tid] = vnext;
+        lds_barrier();
+        const float *S = ring + slot * SLOT;
+        const float own = S[iOwn], hx = S[iHX], hy = S[iHY], fv = S[iF];
+        const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
+        const float xa = fminf(l.uxm ? hx : pxm, l.uxp ? hx : pxp);
+        const float ya = fminf(l.uym ? hy : pym, l.uyp ? hy : pyp);
+        const float za = fminf(vprev, vnext);
+        float nv = vcur;
+        if (p >= plo && p < phi && p != psrc) {
+            float qv;
+            if (UNI) qv = eik3d_f32_uniform(xa, ya, za, fv);
+            else qv = eik3d_f32_general(xa, ya, za, fv, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
+            if (qv < vcur) {
+                nv = qv;
+                bstore(RF, oS, nv);
+                changed = true;
+            }
+        }
+        vprev = nv;
+        vcur = vnext;
+        vnext = (unsigned)(p + 2 - l.pipj) < (unsigned)nz ? own : INF;
+        oS += l.step;
+        slot = slot == R - 1 ? 0 : slot + 1;
+    }
+    const unsigned long long bm = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+}
+
+template <int TI, int TJ, int C, int A, bool S3POS>
+__global__ __launch_bounds__(TI *TJ *C + 64) void adj3d_dma(SlabArgs a, int D, int s1, int s2)
+{
+    static_assert(C == 4, "one dwordx4 lane per column");
+    constexpr int s3 = S3POS ? 1 : -1;
+    constexpr int NT = TI * TJ * C, NCOL = TI * TJ, NH = 2 * (TI + TJ);
+    static_assert(NCOL <= 64 && NH <= 64, "one loader lane per column / halo node");
+    constexpr int R = A + 1;
+    // per slot: own lambda [NT] | own T [NT] | halo lambda [NH*C] | halo T [NH*C]
+    constexpr int S_OT = NT, S_HL = 2 * NT, S_HT = 2 * NT + NH * C, SLOT = 2 * NT + 2 * NH * C;
+    constexpr int PB = 2 * NT + 1;
+    __shared__ float2 pub[2 * PB];
+    __shared__ __attribute__((aligned(16))) float ring[R * SLOT];
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
+    const size_t cb = (size_t)t.chunk * plane * nz * C;
+    const rsrc_t RL = make_rsrc(a.field + cb, 4u * pC * nz);
+    const rsrc_t RT = make_rsrc(a.tfield + cb, 4u * pC * nz);
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
+    const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
+    const unsigned NONE = 0x80000000u;
+    const int pend = nz + t.span;
+    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
+    __syncthreads();
+
+    if (tid >= NT) {  // ------------------------------------------ loader wave
+        const int l = tid - NT;
+        unsigned oOwn = NONE, oH = NONE;
+        if (l < NCOL) {
+            const int li = l / TJ, lj = l % TJ;
+            if (li < t.TIe && lj < t.TJe) {
+                const int pp = pipj_of<TI, TJ>(li, lj, t, s1, s2);
+                const unsigned node = (unsigned)(t.i0 + li) * a.ncols + t.j0 + lj;
+                oOwn = 4u * (k0 * pC + node * C) + (unsigned)(2 - pp) * st;
+            }
+        }
+        if (l < NH) {
+            int cli, clj;
+            unsigned hn;
+            if (halo_entry<TI, TJ>(l, t, a.nrows, a.ncols, cli, clj, hn))
+                oH = 4u * (k0 * pC + hn * C) + (unsigned)(-pipj_of<TI, TJ>(cli, clj, t, s1, s2)) * st;
+        }
+        auto issue = [&](int q) {  // four VMEM instructions (lane 0 is in both groups)
+            float *S = ring + (q % R) * SLOT;
+            if (l < NCOL) {
+                dma16(RL, S, oOwn);
+                dma16(RT, S + S_OT, oOwn);
+            }
+            if (l < NH) {
+                dma16(RL, S + S_HL, oH);
+                dma16(RT, S + S_HT, oH);
+            }
+            oOwn += st;
+            oH += st;
+        };
+        for (int q = 0; q < A; q++) issue(q);
+        for (int p = 0; p < pend; p++) {
+            wait_vm<4 * (A - 1)>();
+            __builtin_amdgcn_s_barrier();
+            issue(p + A);
+        }
+        wait_vm<0>();
+        return;
+    }
+
+    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    const bool col_interior =
+        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
+    const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
+    const float wx = a.wx, wy = a.wy, wz = a.wz;
+    const int iHX = (l.li * 2 + (l.uxp ? 1 : 0)) * C + l.c;
+    const int iHY = (2 * TI + l.lj * 2 + (l.uyp ? 1 : 0)) * C + l.c;
+    auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
+    const int m0 = -l.pipj;
+    float2 vprev = make_float2(0.0f, 0.0f);
+    float2 vcur = l.valid && m0 == 0 ? ld2(l.oI) : make_float2(0.0f, 0.0f);
+    float2 vnext = l.valid && m0 >= -1 ? ld2(l.oI + (unsigned)(m0 + 1) * l.step) : make_float2(0.0f, 0.0f);
+    unsigned oS = l.oI + (unsigned)m0 * l.step;
+    bool changed = false;
+    int slot = 0;
+    for (int p = 0; p < pend; p++) {
+        float2 *P = pub + (p & 1) * PB;
+        P[tid] = vprev;
+        P[NT + tid] = vnext;
+        lds_barrier();
+        const float *S = ring + slot * SLOT;
+        const float2 own = make_float2(S[tid], S[S_OT + tid]);
+        const float2 hx = make_float2(S[S_HL + iHX], S[S_HT + iHX]);
+        const float2 hy = make_float2(S[S_HL + iHY], S[S_HT + iHY]);
+        const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
+        float nl = vcur.x;
+        if (p >= plo && p < phi && !signbit(vcur.y)) {
+            const float2 xm = l.uxm ? hx : pxm;
+            const float2 xp = l.uxp ? hx : pxp;
+            const float2 ym = l.uym ? hy : pym;
+            const float2 yp = l.uyp ? hy : pyp;
+            // depth k-1 / k+1 by absolute depth (see adj3d_col)
+            const float2 zm = S3POS ? vprev : vnext;
+            const float2 zp = S3POS ? vnext : vprev;
+            const float tc = vcur.y;
+            const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
+                        typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
+            float nw = 0.0f;
+            const float I = PFAST_INF_F;
+            if (!(tc >= I || txm >= I || txp >= I || tym >= I || typ >= I || tzm >= I || tzp >= I)) {
+                float den = 0.0f;
+                float num = adj_term(tc, txp, xp.x, wx, den);
+                num += adj_term(tc, txm, xm.x, wx, den);
+                num += adj_term(tc, typ, yp.x, wy, den);
+                num += adj_term(tc, tym, ym.x, wy, den);
+                num += adj_term(tc, tzp, zp.x, wz, den);
+                num += adj_term(tc, tzm, zm.x, wz, den);
+                nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+            }
+            if (nw != vcur.x) {
+                bstore(RL, oS, nw);
+                changed = true;
+            }
+            nl = nw;
+        }
+        vprev = make_float2(nl, vcur.y);
+        vcur = vnext;
+        vnext = own;  // out-of-column levels read 0 and are never used by interior nodes
+        oS += l.step;
+        slot = slot == R - 1 ? 0 : slot + 1;
+    }
+    const unsigned long long bm = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+}
+
 // ------------------------------------------------------------------- helpers
 // T = INF everywhere, 0 at each source's node (padding sources have node -1).
 template <int C>
@@ -904,12 +1232,12 @@ namespace pfast {
 // Sweep kernel variants (schedule, tile rows x columns, batch/prefetch depth).
 // Variant 0 is the default; the others stay selectable for measurement.
 static const SweepVariant kVariants[] = {
-    {0, 8, 8, 4, "column-window 8x8 L4"},
-    {1, 8, 8, 2, "slab-ring 8x8 L2"},
-    {0, 4, 4, 4, "column-window 4x4 L4"},
-    {0, 8, 4, 4, "column-window 8x4 L4"},
-    {0, 8, 8, 2, "column-window 8x8 L2"},
     {0, 8, 8, 6, "column-window 8x8 L6"},
+    {1, 8, 8, 2, "slab-ring 8x8 L2"},
+    {0, 8, 8, 4, "column-window 8x8 L4"},
+    {0, 8, 8, 2, "column-window 8x8 L2"},
+    {0, 8, 4, 4, "column-window 8x4 L4"},
+    {2, 8, 8, 12, "loader-wave 8x8 A12/8"},
 };
 
 int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
@@ -934,6 +1262,25 @@ void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int
         hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2, s3);
     }
 }
+
+// loader-wave kernels: forward lookahead AF steps, adjoint AA steps
+template <int TI, int TJ, int AF, int AA>
+void launch_dma(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
+                int s2, int s3, dim3 grid)
+{
+    constexpr int C = PFAST_CHUNK;
+    const dim3 block(TI * TJ * C + 64);
+    if (adjoint) {
+        if (s3 > 0)
+            hipLaunchKernelGGL((adj3d_dma<TI, TJ, C, AA, true>), grid, block, 0, st, a, D, s1, s2);
+        else
+            hipLaunchKernelGGL((adj3d_dma<TI, TJ, C, AA, false>), grid, block, 0, st, a, D, s1, s2);
+    } else if (uniform) {
+        hipLaunchKernelGGL((fwd3d_dma<TI, TJ, C, AF, true>), grid, block, 0, st, a, D, s1, s2, s3);
+    } else {
+        hipLaunchKernelGGL((fwd3d_dma<TI, TJ, C, AF, false>), grid, block, 0, st, a, D, s1, s2, s3);
+    }
+}
 }  // namespace
 
 hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
@@ -942,7 +1289,7 @@ hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int ad
     constexpr int C = PFAST_CHUNK;
     const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));
     switch (variant) {
-    case 0: launch_col<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 0: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 1: {
         constexpr int TI = 8, TJ = 8, L = 2;
         const dim3 block(TI * TJ * C);
@@ -956,10 +1303,10 @@ hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int ad
                                s2, s3);
         break;
     }
-    case 2: launch_col<4, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 3: launch_col<8, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 4: launch_col<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 5: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 2: launch_col<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 3: launch_col<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 4: launch_col<8, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 5: launch_dma<8, 8, 12, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
