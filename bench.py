@@ -92,6 +92,22 @@ def cpu_baseline(cfg, nthreads, nsamp, maxswp):
                        f"block), wall {wall:.1f} s")
 
 
+def measured_traffic(kernel, sweep_name, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/traffic.json, copy-calibrated), when they were taken
+    on this same workload and sweep variant; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if (t.get("sweep") != sweep_name or t.get("nsrc") != args.nsrc or t.get("nrec") != args.nrec
+            or t.get("grid") != [101, 201, 201]):
+        return None
+    return t.get("bytes_per_launch", {}).get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +227,8 @@ def main():
     else:
         achieved, launches, avg_ms, per_launch = fwd_gbs, fwd_n, fwd_ms / fwd_n, fwd_bytes / fwd_n
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom, sweep_name, args),
                 "kernel": dom, "launches": launches, "avg_launch_ms": round(avg_ms, 5),
                 "bytes_per_launch": round(per_launch)}
     out = {
