@@ -145,7 +145,8 @@ int pfast_shots3d_plain_source(pfast_shots3d *s, int srcid, int MAXSwp, double *
 
 /* Profiling: accumulated device time (ms) and launch count of the dominant sweep
  * kernels of this context since the last reset, measured with HIP events on the
- * context's stream.  kind 0 = forward sweeps, 1 = adjoint sweeps. */
+ * context's stream.  kind 0 = forward sweeps, 1 = adjoint sweeps (3D), 2 = 2D
+ * shot-set solves (forward and adjoints). */
 int pfast_ctx_timing(pfast_ctx *ctx, int kind, double *ms, long *launches);
 /* Node-sweeps (grid nodes x real sources x orderings) issued by the batched
  * sweeps of `kind` since the last timing reset: the work the timed launches did. */
@@ -158,6 +159,43 @@ int pfast_ctx_timing_reset(pfast_ctx *ctx);
  * past the last one. */
 int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant);
 const char *pfast_sweep_name(int variant);
+
+/* ---------------------------------------------------------------------------------
+ * Batched 2D shot sets: every source of a 2D survey resident on one GPU (FP32
+ * sweeps, FP64 reductions).  Replaces the per-source loop of the 2D driver,
+ * src/RT_PFAST2D.cpp:420-435 (RT_fsm2d, Tadjoint_fsm2d, Radjoint_fsm2d, the
+ * -lambda/v^3 accumulations) and its MPI sums.
+ * ------------------------------------------------------------------------------- */
+typedef struct pfast_model2d pfast_model2d;
+typedef struct pfast_shots2d pfast_shots2d;
+
+/* vp: nrows*ncols doubles (reference layout); Z: NumofR*ncols*3 reflector table
+ * (depth, n_z, n_x) as read from the .ref file (NULL when NumofR = 0). */
+int pfast_model2d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols, double dx,
+                         double dz, const double *Z, int NumofR, pfast_model2d **out);
+int pfast_model2d_set_vp(pfast_model2d *m, const double *vp);
+int pfast_model2d_destroy(pfast_model2d *m);
+/* src: 2*nsrc (row, column); rec: 2*nrec; nxy: 2*nrec normals (n_z, n_x);
+ * TT_obs: nrec transmission picks (used where < 100); TR_obs: nrec*NumofR
+ * reflection picks (used where != 0).  rec..TR_obs may be NULL for forward-only use. */
+int pfast_shots2d_create(pfast_model2d *m, int nsrc, const int *src, const int *srnum,
+                         const int *rec, const double *nxy, const double *TT_obs,
+                         const double *TR_obs, pfast_shots2d **out);
+int pfast_shots2d_destroy(pfast_shots2d *s);
+/* RT_fsm2d for every source, then (PFAST_FWD_ADJ) both adjoints.  A source that
+ * hits one of the reference's fatal exits makes the call return PFAST_ERR_FATAL. */
+int pfast_shots2d_run(pfast_shots2d *s, int mode, int MAXSwp);
+/* gT, gR: sum over sources of -lambda/v^3 (transmission, reflection); gRT =
+ * gT (1 - WR) + gR WR with the per-node weights WR (NULL = 0); E3 = {Et, Er, E}.
+ * Any output may be NULL. */
+int pfast_shots2d_gradient(pfast_shots2d *s, const double *WR, double *gT, double *gR,
+                           double *gRT, double *E3);
+/* which: 0 transmission T, 1 TTd / 2 TTu of reflector ir, 3 transmission adjoint,
+ * 4 reflection adjoint (summed over reflectors); N doubles. */
+int pfast_shots2d_field(pfast_shots2d *s, int which, int src, int ir, double *field);
+/* cycles: per source 1+2R forward counts, 1 transmission-adjoint count, R
+ * reflection-adjoint counts; k2vec: per source ncols ints (last reflector). */
+int pfast_shots2d_info(pfast_shots2d *s, int *cycles, int *k2vec);
 
 /* ---------------------------------------------------------------------------------
  * Optimiser support (host arrays), used by the reference-style drivers.

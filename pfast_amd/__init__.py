@@ -27,7 +27,7 @@ from ._lib import (PFAST_FWD_ADJ, PFAST_FWD_ONLY, FatalError, PfastError, check,
 INF = 9999999.0
 
 __all__ = ["fsm3d", "adjoint_fsm3d", "RT_fsm2d", "Tadjoint_fsm2d", "Radjoint_fsm2d", "sweep_variants",
-           "Context", "Model3D", "Shots3D", "device_count", "PfastError", "FatalError", "INF"]
+           "Context", "Model3D", "Shots3D", "Model2D", "Shots2D", "device_count", "PfastError", "FatalError", "INF"]
 
 
 def _f64(a):
@@ -288,6 +288,111 @@ class Shots3D:
     def close(self):
         if self.h:
             lib().pfast_shots3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model2D:
+    """A 2D velocity model resident in HBM (vp shaped (nrows, ncols)) with its
+    reflectors Z shaped (NumofR, ncols, 3): depth, n_z, n_x (the .ref table)."""
+
+    def __init__(self, ctx, vp, dx, dz, Z=None):
+        vp = _f64(vp)
+        self.ctx = ctx
+        self.shape = vp.shape
+        self.dx, self.dz = float(dx), float(dz)
+        nrows, ncols = vp.shape
+        self.NumofR = 0 if Z is None else int(np.asarray(Z).shape[0])
+        self._Z = None if Z is None else _f64(Z).reshape(self.NumofR, ncols, 3)
+        h = C.c_void_p()
+        check(lib().pfast_model2d_create(ctx.h, vp.ravel(), nrows, ncols, self.dx, self.dz,
+                                         ptr(self._Z), self.NumofR, C.byref(h)))
+        self.h = h
+        self._children = weakref.WeakSet()
+        ctx._children.add(self)
+
+    def set_vp(self, vp):
+        vp = _f64(vp)
+        assert vp.shape == self.shape
+        check(lib().pfast_model2d_set_vp(self.h, vp.ravel()))
+
+    def close(self):
+        if self.h:
+            for s in list(self._children):
+                s.close()
+            check(lib().pfast_model2d_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Shots2D:
+    """Every source of a 2D survey on the GPU: RT_fsm2d plus both adjoints per source
+    in FP32, FP64 gradient and misfit reductions (the 2D driver's per-source loop).
+
+    src: (nsrc, 2) nodes (row, col); srnum: receivers per source; rec: (nrec, 2);
+    nxy: (nrec, 2) normals (n_z, n_x); TT_obs: (nrec,) transmission picks (< 100 used);
+    TR_obs: (nrec, NumofR) reflection picks (!= 0 used).
+    """
+
+    def __init__(self, model, src, srnum, rec=None, nxy=None, TT_obs=None, TR_obs=None):
+        self.model = model
+        self.src = _i32(src).reshape(-1, 2)
+        self.nsrc = self.src.shape[0]
+        self.srnum = _i32(srnum)
+        self.nrec = int(self.srnum.sum())
+        R = model.NumofR
+        self._rec = None if rec is None else _i32(rec).reshape(-1, 2)
+        self._nxy = None if nxy is None else _f64(nxy).reshape(-1, 2)
+        self._tt = None if TT_obs is None else _f64(TT_obs)
+        self._tr = None if TR_obs is None else _f64(TR_obs).reshape(-1, max(R, 1))
+        h = C.c_void_p()
+        check(lib().pfast_shots2d_create(model.h, self.nsrc, self.src.ravel(), self.srnum,
+                                         ptr(self._rec), ptr(self._nxy), ptr(self._tt),
+                                         ptr(self._tr), C.byref(h)))
+        self.h = h
+        model._children.add(self)
+
+    def run(self, adjoint=True, MAXSwp=500):
+        check(lib().pfast_shots2d_run(self.h, PFAST_FWD_ADJ if adjoint else PFAST_FWD_ONLY,
+                                      int(MAXSwp)))
+
+    def gradient(self, WR=None):
+        """(gT, gR, gRT, (Et, Er, E)); gRT = gT (1 - WR) + gR WR (WR per node or None)."""
+        N = int(np.prod(self.model.shape))
+        gT, gR, gRT, E3 = np.empty(N), np.empty(N), np.empty(N), np.empty(3)
+        wr = None if WR is None else _f64(np.broadcast_to(WR, self.model.shape))
+        check(lib().pfast_shots2d_gradient(self.h, ptr(wr), ptr(gT), ptr(gR), ptr(gRT), ptr(E3)))
+        sh = self.model.shape
+        return gT.reshape(sh), gR.reshape(sh), gRT.reshape(sh), tuple(E3)
+
+    def field(self, s, which="T", ir=0):
+        """which: 'T', 'TTd', 'TTu', 'lambdaT', 'lambdaR'."""
+        k = {"T": 0, "TTd": 1, "TTu": 2, "lambdaT": 3, "lambdaR": 4}[which]
+        out = np.empty(int(np.prod(self.model.shape)))
+        check(lib().pfast_shots2d_field(self.h, k, int(s), int(ir), out))
+        return out.reshape(self.model.shape)
+
+    def info(self):
+        """(cycles (nsrc, 1+2R+1+R), k2vec (nsrc, ncols))."""
+        R = self.model.NumofR
+        cyc = np.empty((self.nsrc, 2 + 3 * R), np.int32)
+        k2 = np.empty((self.nsrc, self.model.shape[1]), np.int32)
+        check(lib().pfast_shots2d_info(self.h, ptr(cyc), ptr(k2)))
+        return cyc, k2
+
+    def close(self):
+        if self.h:
+            lib().pfast_shots2d_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -63,6 +63,15 @@ _SIGS = {
     "pfast_shots3d_cycles": [vpp, ip],
     "pfast_shots3d_plain_source": [vpp, i, i, pdbl, vpp, vpp, vpp],
     "pfast_ctx_work": [vpp, i, pdbl],
+    "pfast_model2d_create": [vpp, dp, i, i, d, d, vpp, i, C.POINTER(vpp)],
+    "pfast_model2d_set_vp": [vpp, dp],
+    "pfast_model2d_destroy": [vpp],
+    "pfast_shots2d_create": [vpp, i, ip, ip, vpp, vpp, vpp, vpp, C.POINTER(vpp)],
+    "pfast_shots2d_destroy": [vpp],
+    "pfast_shots2d_run": [vpp, i, i],
+    "pfast_shots2d_gradient": [vpp, vpp, vpp, vpp, vpp, vpp],
+    "pfast_shots2d_field": [vpp, i, i, i, dp],
+    "pfast_shots2d_info": [vpp, vpp, vpp],
     "pfast_regu3d": [dp, dp, d, d, d, d, d, d, i, i, i],
     "pfast_regu2d": [dp, dp, d, d, d, d, i, i],
 }

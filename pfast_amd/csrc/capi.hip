@@ -13,6 +13,7 @@
 
 #include "../../include/pfast.h"
 #include "batch3d.h"
+#include "context.h"
 #include "pfast_internal.h"
 #include "plain_kernels.h"
 
@@ -22,21 +23,6 @@ void set_error(const std::string &msg) { g_err = msg; }
 }  // namespace pfast
 
 // ------------------------------------------------------------------ context
-struct pfast_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool timing = false;
-    double ms[2] = {0, 0};
-    long launches[2] = {0, 0};
-    double units[2] = {0, 0};  // node-sweeps of real sources issued (batched path)
-    int sweep = 0;             // batched sweep kernel variant
-    int nmodels = 0;           // live models bound to this context
-    std::vector<hipEvent_t> evpool;
-    std::vector<std::pair<int, int>> pending;  // (kind, event index of start)
-    size_t evnext = 0;
-    // grow-only device scratch for the single-source forms
-    std::map<std::string, std::pair<void *, size_t>> cache;
-};
 
 namespace {
 
@@ -92,7 +78,9 @@ int default_ctx(pfast_ctx **out)
     return PFAST_OK;
 }
 
-// timing helpers: an event pair around one launch when enabled
+}  // namespace
+
+namespace pfast {
 void time_begin(pfast_ctx *ctx, int kind)
 {
     if (!ctx->timing) return;
@@ -129,7 +117,7 @@ void time_collect(pfast_ctx *ctx)
     ctx->evnext = 0;
 }
 
-}  // namespace
+}  // namespace pfast
 
 extern "C" {
 
@@ -181,7 +169,7 @@ int pfast_ctx_sync(pfast_ctx *ctx)
 {
     ARGCHK(ctx, "ctx is NULL");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    time_collect(ctx);
+    pfast::time_collect(ctx);
     return PFAST_OK;
 }
 
@@ -212,24 +200,26 @@ int pfast_ctx_timing_reset(pfast_ctx *ctx)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->pending.clear();
     ctx->evnext = 0;
-    ctx->ms[0] = ctx->ms[1] = 0;
-    ctx->launches[0] = ctx->launches[1] = 0;
-    ctx->units[0] = ctx->units[1] = 0;
+    for (int k = 0; k < pfast::kTimedKinds; k++) {
+        ctx->ms[k] = 0;
+        ctx->launches[k] = 0;
+        ctx->units[k] = 0;
+    }
     return PFAST_OK;
 }
 
 int pfast_ctx_work(pfast_ctx *ctx, int kind, double *node_sweeps)
 {
-    ARGCHK(ctx && kind >= 0 && kind < 2 && node_sweeps, "bad argument");
+    ARGCHK(ctx && kind >= 0 && kind < pfast::kTimedKinds && node_sweeps, "bad argument");
     *node_sweeps = ctx->units[kind];
     return PFAST_OK;
 }
 
 int pfast_ctx_timing(pfast_ctx *ctx, int kind, double *ms, long *launches)
 {
-    ARGCHK(ctx && kind >= 0 && kind < 2, "bad ctx/kind");
+    ARGCHK(ctx && kind >= 0 && kind < pfast::kTimedKinds, "bad ctx/kind");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    time_collect(ctx);
+    pfast::time_collect(ctx);
     if (ms) *ms = ctx->ms[kind];
     if (launches) *launches = ctx->launches[kind];
     return PFAST_OK;
@@ -256,15 +246,15 @@ int fsm3d_dev(pfast_ctx *ctx, const double *vp, double *U, double *p, int *flag,
         HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), st));
         for (int o = 0; o < 8; o++) {
             const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
-            time_begin(ctx, 0);
+            pfast::time_begin(ctx, 0);
             HIPCHK(pfast::launch_fsm3d_ordering_f64(st, vp, U, p, flag, si, sj, sk, s1, s2, s3, dx,
                                                     dy, dz, nrows, ncols, nz, o == 0, o == 7));
-            time_end(ctx);
+            pfast::time_end(ctx);
         }
         int h = 0;
         HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        time_collect(ctx);
+        pfast::time_collect(ctx);
         if (!h) break;
     }
     if (cycles_out) *cycles_out = count;
@@ -298,14 +288,14 @@ int adj3d_dev(pfast_ctx *ctx, double *U, const double *T, unsigned char *tag, do
         HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), st));
         for (int o = 0; o < 8; o++) {
             const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
-            time_begin(ctx, 1);
+            pfast::time_begin(ctx, 1);
             HIPCHK(pfast::launch_adj3d_ordering_f64(st, U, T, tag, p, flag, s1, s2, s3, dx, dy, dz,
                                                     nrows, ncols, nz, o == 0, o == 7));
-            time_end(ctx);
+            pfast::time_end(ctx);
         }
         HIPCHK(hipMemcpyAsync(&notconv, flag, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        time_collect(ctx);
+        pfast::time_collect(ctx);
     }
     if (EE_out) *EE_out = EE;
     if (cycles_out) *cycles_out = swept;
@@ -593,25 +583,6 @@ struct pfast_shots3d {
 
 namespace {
 
-template <class T>
-int dalloc(T **p, size_t count)
-{
-    if (hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
-        pfast::set_error("hipMalloc failed (" + std::to_string(count * sizeof(T)) + " bytes)");
-        return PFAST_ERR_NOMEM;
-    }
-    return PFAST_OK;
-}
-
-template <class T>
-int dupload(T **p, const T *h, size_t count, hipStream_t st)
-{
-    int rc = dalloc(p, count);
-    if (rc) return rc;
-    if (count && h) HIPCHK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
-    return PFAST_OK;
-}
-
 // Sweep cycles of the batched FP32 solver until every source is stationary.
 int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &cyc)
 {
@@ -656,15 +627,15 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
             const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
             for (int D = 0; D <= nti + ntj - 2; D++) {
                 const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
-                time_begin(ctx, adjoint ? 1 : 0);
+                pfast::time_begin(ctx, adjoint ? 1 : 0);
                 HIPCHK(pfast::launch_sweep3d(st, a, ctx->sweep, adjoint, m->uniform, D, s1, s2, s3,
                                               hi - lo + 1));
-                time_end(ctx);
+                pfast::time_end(ctx);
             }
         }
         HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        time_collect(ctx);
+        pfast::time_collect(ctx);
         std::vector<int> next;
         for (int ch : act) {
             const unsigned bits = s->hchanged[ch];
@@ -704,7 +675,7 @@ int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     m->uniform = dx == dy && dy == dz;
     const long N = (long)nrows * ncols * nz;
     int rc;
-    if ((rc = dalloc(&m->vp, N)) || (rc = dalloc(&m->f, N))) {
+    if ((rc = pfast::dalloc(&m->vp, N)) || (rc = pfast::dalloc(&m->f, N))) {
         pfast_model3d_destroy(m);
         return rc;
     }
@@ -799,10 +770,10 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         sn[i] = (int)(((long)src[3 * i + 2] * nrows + src[3 * i]) * ncols + src[3 * i + 1]);
     int rc = 0;
     const size_t fsz = (size_t)s->nchunk * N * PFAST_CHUNK;
-    if ((rc = dupload(&s->srcnode, sn.data(), sn.size(), st)) || (rc = dalloc(&s->T, fsz)) ||
-        (rc = dalloc(&s->changed, s->nchunk)) || (rc = dalloc(&s->active, s->nchunk)) ||
-        (rc = dalloc(&s->EE, nsrc)) || (rc = dalloc(&s->grad, N)) ||
-        (rc = dalloc(&s->fieldbuf, N))) {
+    if ((rc = pfast::dupload(&s->srcnode, sn.data(), sn.size(), st)) || (rc = pfast::dalloc(&s->T, fsz)) ||
+        (rc = pfast::dalloc(&s->changed, s->nchunk)) || (rc = pfast::dalloc(&s->active, s->nchunk)) ||
+        (rc = pfast::dalloc(&s->EE, nsrc)) || (rc = pfast::dalloc(&s->grad, N)) ||
+        (rc = pfast::dalloc(&s->fieldbuf, N))) {
         pfast_shots3d_destroy(s);
         return rc;
     }
@@ -821,9 +792,9 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
                 rnode[r] = ((long)q[2] * nrows + q[0]) * ncols + q[1];
                 rsrc[r] = i;
             }
-        if ((rc = dupload(&s->rnode, rnode.data(), s->nrec, st)) ||
-            (rc = dupload(&s->rsrc, rsrc.data(), s->nrec, st)) ||
-            (rc = dupload(&s->rec3, rec, 3 * s->nrec, st)) || (rc = dalloc(&s->picks, s->nrec))) {
+        if ((rc = pfast::dupload(&s->rnode, rnode.data(), s->nrec, st)) ||
+            (rc = pfast::dupload(&s->rsrc, rsrc.data(), s->nrec, st)) ||
+            (rc = pfast::dupload(&s->rec3, rec, 3 * s->nrec, st)) || (rc = pfast::dalloc(&s->picks, s->nrec))) {
             pfast_shots3d_destroy(s);
             return rc;
         }
@@ -856,16 +827,16 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         }
         s->npick = (int)pnode.size();
         s->nb = (int)bnode.size();
-        if ((rc = dalloc(&s->L, fsz)) || (rc = dupload(&s->pnode, pnode.data(), pnode.size(), st)) ||
-            (rc = dupload(&s->psrc, psrc.data(), psrc.size(), st)) ||
-            (rc = dupload(&s->pobs, pobs.data(), pobs.size(), st)) ||
-            (rc = dupload(&s->poff, poff.data(), poff.size(), st)) ||
-            (rc = dupload(&s->bnode, bnode.data(), bnode.size(), st)) ||
-            (rc = dupload(&s->bsrc, bsrc.data(), bsrc.size(), st)) ||
-            (rc = dupload(&s->bobs, bobs.data(), bobs.size(), st)) ||
-            (rc = dupload(&s->bnrm, bnrm.data(), bnrm.size(), st)) ||
-            (rc = dupload(&s->nxyz, nxyz, 3 * s->nrec, st)) ||
-            (rc = dupload(&s->Tobs, T_obs, s->nrec, st))) {
+        if ((rc = pfast::dalloc(&s->L, fsz)) || (rc = pfast::dupload(&s->pnode, pnode.data(), pnode.size(), st)) ||
+            (rc = pfast::dupload(&s->psrc, psrc.data(), psrc.size(), st)) ||
+            (rc = pfast::dupload(&s->pobs, pobs.data(), pobs.size(), st)) ||
+            (rc = pfast::dupload(&s->poff, poff.data(), poff.size(), st)) ||
+            (rc = pfast::dupload(&s->bnode, bnode.data(), bnode.size(), st)) ||
+            (rc = pfast::dupload(&s->bsrc, bsrc.data(), bsrc.size(), st)) ||
+            (rc = pfast::dupload(&s->bobs, bobs.data(), bobs.size(), st)) ||
+            (rc = pfast::dupload(&s->bnrm, bnrm.data(), bnrm.size(), st)) ||
+            (rc = pfast::dupload(&s->nxyz, nxyz, 3 * s->nrec, st)) ||
+            (rc = pfast::dupload(&s->Tobs, T_obs, s->nrec, st))) {
             pfast_shots3d_destroy(s);
             return rc;
         }

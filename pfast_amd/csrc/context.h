@@ -1,0 +1,60 @@
+// context.h -- the library context (device stream, profiling state, scratch) and
+// the device-allocation helpers shared by the host translation units of libpfast.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pfast.h"
+#include "pfast_internal.h"
+
+struct pfast_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    double ms[3] = {0, 0, 0};     // per timed kind: forward sweeps, adjoint sweeps, 2D solves
+    long launches[3] = {0, 0, 0};
+    double units[3] = {0, 0, 0};  // node-sweeps of real sources issued (batched paths)
+    int sweep = 0;                // batched sweep kernel variant
+    int nmodels = 0;              // live models bound to this context
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, int>> pending;  // (kind, event index of start)
+    size_t evnext = 0;
+    // grow-only device scratch for the single-source forms
+    std::map<std::string, std::pair<void *, size_t>> cache;
+};
+
+namespace pfast {
+
+constexpr int kTimedKinds = 3;
+
+// An event pair around one launch when timing is enabled; folded into the
+// totals by time_collect after a stream synchronisation.
+void time_begin(pfast_ctx *ctx, int kind);
+void time_end(pfast_ctx *ctx);
+void time_collect(pfast_ctx *ctx);
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    if (hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+        set_error("hipMalloc failed (" + std::to_string(count * sizeof(T)) + " bytes)");
+        return PFAST_ERR_NOMEM;
+    }
+    return PFAST_OK;
+}
+
+template <class T>
+int dupload(T **p, const T *h, size_t count, hipStream_t st)
+{
+    int rc = dalloc(p, count);
+    if (rc) return rc;
+    if (count && h) HIPCHK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return PFAST_OK;
+}
+
+}  // namespace pfast
