@@ -1,0 +1,191 @@
+"""Driver trajectories of the reference's own tomography drivers, as fixtures.
+
+Needs the reference drivers, built from the reference sources under /root/reference
+by `make -C oracle ref-drivers`:
+
+    python tests/golden/gen_driver_golden.py
+
+For a small seeded 2D joint transmission/reflection survey and a small 3D survey
+it writes the reference file formats (parameter file, .vp/.src/.rec/.ref/.wr),
+runs the reference drivers (oracle/_ref/RT_PFAST2D, oracle/_ref/T_PFAST3D, FP64 on
+the CPU, one MPI rank) and keeps, in driver2d.npz / driver3d.npz:
+  inputs      -- the starting model, source / receiver / reflector tables, weights
+  E, Et, Er   -- the misfit per iteration (the reference's _GlobalErr files)
+  steps       -- the accepted step size per iteration (its log)
+  trials      -- every line-search trial (alpha0, alpha1, phi0, phi1, dphi0, dphi1)
+  E_ens, steps_ens -- the same run from NENS starting models perturbed by 1e-5
+                 relative (seeded normal noise), NaN-padded: the reference's own
+                 sensitivity, which bounds how closely any other arithmetic can
+                 follow its trajectory
+  vp1, vpF    -- the model after iteration 1 and the final model
+Observed picks come from the FP64 oracle on a seeded "true" model plus 2 ms noise.
+"""
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+import oracle_lib as O  # noqa: E402
+from pfast_amd import formats as F  # noqa: E402
+
+REFBIN = os.path.join(ROOT, "oracle", "_ref")
+NENS, PERTURB = 4, 1e-5
+
+
+def case2d(seed=2024):
+    rng = np.random.default_rng(seed)
+    nr, nc, dx, dz = 41, 81, 12.5, 8.0
+    z = (np.arange(nr) * dz)[:, None]
+    x = (np.arange(nc) * dx)[None, :]
+    v0 = np.broadcast_to(1700.0 + 1.2 * z, (nr, nc)).copy()
+    vt = v0 * (1 + 0.08 * np.exp(-((x - 500) ** 2 + (z - 140) ** 2) / (2 * 80.0 ** 2)))
+    R = 1
+    Zd = 260.0 + 12.0 * np.sin(2 * np.pi * np.arange(nc) / nc)
+    Z = np.stack([Zd, -np.ones(nc), np.zeros(nc)], 1)
+    nsrc, nrec = 8, 40
+    sx = (np.linspace(3, nc - 4, nsrc).round() + 1) * dx
+    srcA = np.stack([np.full(nsrc, dz), sx, np.full(nsrc, nrec)], 1)
+    rx = (np.linspace(1, nc - 2, nrec).round() + 1) * dx
+    src = np.stack([F.map_index(srcA[:, 0], dz), F.map_index(srcA[:, 1], dx)], 1).astype(np.int32)
+    rows = []
+    for s in range(nsrc):
+        o = O.rt_fsm2d(vt, Z[None], src, s, R, dx, dz)
+        assert o["status"] == 0
+        ri = F.map_index(np.full(nrec, dz), dz)
+        rj = F.map_index(rx, dx)
+        tt = o["T"][ri, rj] + rng.normal(0, 0.002, nrec)
+        tr = o["TTu"][0][ri, rj] + rng.normal(0, 0.002, nrec)
+        rows.append(np.stack([np.full(nrec, dz), rx, -np.ones(nrec), np.zeros(nrec), tt, tr], 1))
+    recA = np.concatenate(rows)
+    return dict(v0=v0, vt=vt, srcA=srcA, recA=recA, Z=Z, WR=np.full((nr, nc), 0.5), nr=nr, nc=nc,
+                dx=dx, dz=dz)
+
+
+def case3d(seed=2025):
+    rng = np.random.default_rng(seed)
+    nr, nc, nz, h = 15, 17, 11, 25.0
+    z = (np.arange(nz) * h)[:, None, None]
+    y = (np.arange(nr) * h)[None, :, None]
+    x = (np.arange(nc) * h)[None, None, :]
+    v0 = np.broadcast_to(2000.0 + 0.6 * z, (nz, nr, nc)).copy()
+    vt = (2050.0 + 0.9 * z) * (1 + 0.25 * np.exp(-((x - 200) ** 2 + (y - 175) ** 2 + (z - 100) ** 2)
+                                                   / (2 * 70.0 ** 2)))
+    nsrc, nrec = 6, 30
+    sy = rng.integers(1, nr - 1, nsrc) + 1.0
+    sxx = rng.integers(1, nc - 1, nsrc) + 1.0
+    srcA = np.stack([sy * h, sxx * h, np.full(nsrc, h), np.full(nsrc, nrec)], 1)
+    src = np.stack([F.map_index(srcA[:, 0], h), F.map_index(srcA[:, 1], h),
+                    F.map_index(srcA[:, 2], h)], 1).astype(np.int32)
+    rows = []
+    for s in range(nsrc):
+        T, _ = O.fsm3d(vt, src, s, h, h, h)
+        ry = rng.integers(1, nr - 1, nrec) + 1.0
+        rxx = rng.integers(1, nc - 1, nrec) + 1.0
+        ri, rj = F.map_index(ry * h, h), F.map_index(rxx * h, h)
+        t = T[0, ri, rj] + rng.normal(0, 0.002, nrec)
+        rows.append(np.stack([ry * h, rxx * h, np.full(nrec, h), np.zeros(nrec), np.zeros(nrec),
+                              -np.ones(nrec), t], 1))
+    recA = np.concatenate(rows)
+    return dict(v0=v0, vt=vt, srcA=srcA, recA=recA, nr=nr, nc=nc, nz=nz, h=h)
+
+
+def write2d(d, c, schemetag=3, lsrc=2):
+    F.write_bin(os.path.join(d, "m.vp"), c["v0"])
+    F.write_bin(os.path.join(d, "m.src"), c["srcA"])
+    F.write_bin(os.path.join(d, "m.rec"), c["recA"])
+    F.write_bin(os.path.join(d, "m.ref"), c["Z"])
+    F.write_bin(os.path.join(d, "m.wr"), c["WR"])
+    F.write_inp2d(os.path.join(d, "m.inp"), "m.vp", "m.src", c["srcA"].shape, "m.rec",
+                  c["recA"].shape, "m.ref", c["Z"].shape, c["nr"], c["nc"], c["dx"], c["dz"],
+                  vpfile="final.vp", nux=60, nuz=20, twfile="m.wr", schemetag=schemetag, lsrc=lsrc)
+
+
+def write3d(d, c, lsrc=2):
+    F.write_bin(os.path.join(d, "m.vp"), c["v0"])
+    F.write_bin(os.path.join(d, "m.src"), c["srcA"])
+    F.write_bin(os.path.join(d, "m.rec"), c["recA"])
+    F.write_inp3d(os.path.join(d, "m.inp"), "m.vp", "m.src", c["srcA"].shape, "m.rec",
+                  c["recA"].shape, c["nr"], c["nc"], c["nz"], c["h"], c["h"], c["h"],
+                  vpfile="final.vp", nux=50, nuy=50, nuz=25, schemetag=3, lsrc=lsrc)
+
+
+def steps_from_log(log):
+    return [float(v) for v in re.findall(r"Stepsize=([-0-9.eE+]+)", log)]
+
+
+def trials_from_log(log):
+    rows = re.findall(r"\(alpha0,alpha1,phi0,phi1,dphi0,dphi1\)=\(([^\n]*)", log)
+    return np.array([[float(v) for v in r.split()[:6]] for r in rows]).reshape(-1, 6)
+
+
+def run(exe, d, args, timeout):
+    r = subprocess.run([exe, "m.inp", *args], cwd=d, capture_output=True, text=True,
+                       timeout=timeout)
+    return r.returncode, r.stdout
+
+
+def ensemble(c, write, exe, args, errfile, col=None):
+    """Misfit and step sequences of the reference from perturbed starting models."""
+    rng = np.random.default_rng(7)
+    Es, Ss = [], []
+    for _ in range(NENS):
+        cp = dict(c)
+        cp["v0"] = c["v0"] * (1 + PERTURB * rng.standard_normal(c["v0"].shape))
+        with tempfile.TemporaryDirectory() as d:
+            write(d, cp)
+            rc, log = run(exe, d, args, 600)
+            assert rc == 0, log[-2000:]
+            E = np.loadtxt(os.path.join(d, errfile), ndmin=2 if col is not None else 1)
+            Es.append(E[:, col] if col is not None else E)
+            Ss.append(np.array(steps_from_log(log)))
+    n = max(len(e) for e in Es)
+    pad = lambda v: np.concatenate([v, np.full(n - len(v), np.nan)])
+    return np.stack([pad(e) for e in Es]), np.stack([pad(v) for v in Ss])
+
+
+def main():
+    out = {}
+    c = case2d()
+    with tempfile.TemporaryDirectory() as d:
+        write2d(d, c)
+        rc, log = run(os.path.join(REFBIN, "RT_PFAST2D"), d, ["50", "3"], 600)
+        assert rc == 0, log[-2000:]
+        E = np.loadtxt(os.path.join(d, "m_GlobalErr_TR_T_R.txt"), ndmin=2)
+        out2 = dict(v0=c["v0"], srcA=c["srcA"], recA=c["recA"], Z=c["Z"], WR=c["WR"],
+                    grid=np.array([c["nr"], c["nc"]]), spacing=np.array([c["dx"], c["dz"]]),
+                    E=E[:, 0], Et=E[:, 1], Er=E[:, 2], steps=np.array(steps_from_log(log)),
+                    trials=trials_from_log(log),
+                    vp1=F.read_bin(os.path.join(d, "m_1"), (c["nr"], c["nc"])),
+                    vpF=F.read_bin(os.path.join(d, "final.vp"), (c["nr"], c["nc"])))
+        out2["E_ens"], out2["steps_ens"] = ensemble(c, write2d, os.path.join(REFBIN, "RT_PFAST2D"),
+                                                    ["50", "3"], "m_GlobalErr_TR_T_R.txt", col=0)
+        print(f"2D: {len(E)} iterations, E {E[0, 0]:.6g} -> {E[-1, 0]:.6g}, steps "
+              f"{out2['steps'][:4]}")
+        np.savez_compressed(os.path.join(HERE, "driver2d.npz"), **out2)
+    c = case3d()
+    with tempfile.TemporaryDirectory() as d:
+        write3d(d, c)
+        rc, log = run(os.path.join(REFBIN, "T_PFAST3D"), d, ["50"], 600)
+        assert rc == 0, log[-2000:]
+        E = np.loadtxt(os.path.join(d, "m_GlobalErr.txt"), ndmin=1)
+        shape = (c["nz"], c["nr"], c["nc"])
+        out3 = dict(v0=c["v0"], srcA=c["srcA"], recA=c["recA"],
+                    grid=np.array([c["nr"], c["nc"], c["nz"]]), spacing=np.array([c["h"]] * 3),
+                    E=E, steps=np.array(steps_from_log(log)), trials=trials_from_log(log),
+                    vp1=F.read_bin(os.path.join(d, "m.vp1"), shape),
+                    vpF=F.read_bin(os.path.join(d, "final.vp"), shape))
+        out3["E_ens"], out3["steps_ens"] = ensemble(c, write3d, os.path.join(REFBIN, "T_PFAST3D"),
+                                                    ["50"], "m_GlobalErr.txt")
+        print(f"3D: {len(E)} iterations, E {E[0]:.6g} -> {E[-1]:.6g}, steps {out3['steps'][:4]}")
+        np.savez_compressed(os.path.join(HERE, "driver3d.npz"), **out3)
+
+
+if __name__ == "__main__":
+    main()

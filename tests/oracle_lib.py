@@ -63,6 +63,12 @@ def ref3d():
         L.ref_adjoint_fsm3d.restype = d
         L.ref_regu3d.argtypes = [dp, dp, d, d, d, d, d, d, i, i, i]
         L.ref_regu3d.restype = None
+        L.ref_gd2sk3d.argtypes = [dp, dp, dp, d, dp, dp, dp, i, i, i, i, i]
+        L.ref_gd2sk3d.restype = d
+        L.ref_magnitudeOf3d.argtypes = [dp, i, i, i]
+        L.ref_magnitudeOf3d.restype = d
+        L.ref_updatevp3d.argtypes = [dp, dp, d, i, i, i]
+        L.ref_updatevp3d.restype = None
         _ref3 = L
     return _ref3
 
@@ -83,6 +89,18 @@ def ref2d():
         L.ref_radjoint_fsm2d_wr.restype = d
         L.ref_regu2d.argtypes = [dp, dp, d, d, d, d, i, i]
         L.ref_regu2d.restype = None
+        L.ref_beta_HS2d.argtypes = [dp, dp, dp, i, i]
+        L.ref_beta_HS2d.restype = d
+        L.ref_beta_CGDESCENT2d.argtypes = [dp, dp, dp, i, i]
+        L.ref_beta_CGDESCENT2d.restype = d
+        L.ref_gd2sk2d.argtypes = [dp, dp, dp, d, dp, dp, dp, i, i, i, i]
+        L.ref_gd2sk2d.restype = d
+        L.ref_cubicInterp.argtypes = [d, d, d, d, d, d]
+        L.ref_cubicInterp.restype = d
+        L.ref_magnitudeOf2d.argtypes = [dp, i, i]
+        L.ref_magnitudeOf2d.restype = d
+        L.ref_updatevp2d.argtypes = [dp, dp, d, i, i]
+        L.ref_updatevp2d.restype = None
         _ref2 = L
     return _ref2
 
@@ -155,4 +173,32 @@ def map_index(table, col, h):
     n, stride = table.shape
     out = np.empty(n, np.int32)
     oracle().orc_map_index(table.ravel(), n, stride, col, h, out, 1)
+    return out
+
+
+def ref_wavenumbers(n, h):
+    """k[j] = -pi/h + 2 pi j / (n h), then the two halves of length n // 2 swapped
+    (src/T_padjoint3d.h:218-232, src/RT_padjoint2d.h:520-530); for odd n this is a
+    half-bin-shifted permutation that the regularisation reproduces as is."""
+    k = -np.pi / h + 2 * np.pi * np.arange(n) / n / h
+    m = n // 2
+    k[:m], k[m:2 * m] = k[m:2 * m].copy(), k[:m].copy()
+    return k
+
+
+def regu_np(Dj, vp, nus, hs):
+    """FFT smoothing of a gradient (src/T_padjoint3d.h:205-262, 2D :506-545 of
+    src/RT_padjoint2d.h) restated with numpy's FFT: Dj <- Re IFFT[FFT(Dj) /
+    (1 + sum nu_a^2 k_a^2)], air nodes (vp = 330) set to 0.  Axes in array order:
+    3D (z, rows, cols) with nus = (nuz, nuy, nux), hs = (dz, dy, dx); 2D (rows, cols)
+    with nus = (nuz, nux), hs = (dz, dx)."""
+    F = np.fft.fftn(Dj)
+    den = np.ones(Dj.shape)
+    for ax, (nu, h) in enumerate(zip(nus, hs)):
+        k = ref_wavenumbers(Dj.shape[ax], h)
+        sh = [1] * Dj.ndim
+        sh[ax] = -1
+        den = den + (nu * nu * k * k).reshape(sh)
+    out = np.real(np.fft.ifftn(F / den))
+    out[np.abs(vp - 330) < 1e-9] = 0.0
     return out
