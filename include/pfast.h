@@ -222,6 +222,71 @@ double pfast_cubic_interp(double alpha0, double alpha1, double phi0, double phi1
 double pfast_magnitude_of(const double *g, long N);
 double pfast_mean_of(const double *g, long N);
 
+/* ---------------------------------------------------------------------------------
+ * Request server (no counterpart in the reference: its drivers run one survey per
+ * MPI job).  Velocity models stay resident on the device; traveltime-table and
+ * gradient requests (a source set on a resident model) arrive at any time, are
+ * admitted by device-memory footprint against the budget, and run batched: the
+ * requests of one 3D grid shape share every sweep launch, joining and leaving at
+ * sweep-cycle boundaries, so each result is bit-identical to the same request run
+ * alone through pfast_shots3d_run / pfast_shots2d_run.  One host thread drives the
+ * server through pfast_server_progress; calls on one server must not overlap.
+ * ------------------------------------------------------------------------------- */
+typedef struct pfast_server pfast_server;
+
+#define PFAST_REQ_TABLE 1    /* traveltime fields of every source */
+#define PFAST_REQ_GRADIENT 2 /* forward + adjoint: gradient and misfit */
+
+#define PFAST_REQ_QUEUED 0
+#define PFAST_REQ_RUNNING 1
+#define PFAST_REQ_DONE 2
+#define PFAST_REQ_FAILED 3
+
+/* budget_bytes bounds the device memory of resident models and admitted requests. */
+int pfast_server_create(int device, size_t budget_bytes, pfast_server **out);
+int pfast_server_destroy(pfast_server *srv);
+int pfast_server_add_model3d(pfast_server *srv, const double *vp, int nrows, int ncols, int nz,
+                             double dx, double dy, double dz, int *model_id);
+/* Z, NumofR as for pfast_model2d_create. */
+int pfast_server_add_model2d(pfast_server *srv, const double *vp, int nrows, int ncols,
+                             double dx, double dz, const double *Z, int NumofR, int *model_id);
+/* Only once every request on the model is released. */
+int pfast_server_remove_model(pfast_server *srv, int model_id);
+/* Arrays as for pfast_shots3d_create / pfast_shots2d_create, copied at submission.
+ * A table request needs no receivers; a request larger than the budget is refused
+ * (PFAST_ERR_NOMEM).  ticket receives the request id. */
+int pfast_server_submit3d(pfast_server *srv, int model_id, int kind, int nsrc, const int *src,
+                          const int *srnum, const int *rec, const double *nxyz,
+                          const double *T_obs, int MAXSwp, long *ticket);
+int pfast_server_submit2d(pfast_server *srv, int model_id, int kind, int nsrc, const int *src,
+                          const int *srnum, const int *rec, const double *nxy,
+                          const double *TT_obs, const double *TR_obs, int MAXSwp, long *ticket);
+/* Admit, launch and retire work; returns after at least one request completed or
+ * wait_us microseconds (0: one pass).  ncompleted (may be NULL) counts completions. */
+int pfast_server_progress(pfast_server *srv, int wait_us, int *ncompleted);
+/* state: PFAST_REQ_*; times (may be NULL): submission, admission and completion in
+ * seconds since the server was created (-1 when not reached). */
+int pfast_server_status(pfast_server *srv, long ticket, int *state, double *times);
+/* Per source (forward, adjoint) sweep cycles (3D: 2*nsrc ints; 2D: as
+ * pfast_shots2d_info). */
+int pfast_server_cycles(pfast_server *srv, long ticket, int *cycles);
+/* Gradient request: 3D grad (N doubles) and misfit[0]; 2D grad = gT, grad_r = gR
+ * and misfit[0..1] = (Et, Er).  Outputs may be NULL.  A failed request returns its
+ * error. */
+int pfast_server_gradient(pfast_server *srv, long ticket, double *grad, double *grad_r,
+                          double *misfit);
+/* Traveltime field of source src of a finished request (N doubles). */
+int pfast_server_field(pfast_server *srv, long ticket, int src, double *T);
+/* Drop a request: a queued one leaves the queue, a finished one frees its device
+ * memory, a running one is dropped when it finishes. */
+int pfast_server_release(pfast_server *srv, long ticket);
+/* Device memory the request is charged against the budget while admitted. */
+int pfast_server_footprint(pfast_server *srv, long ticket, size_t *bytes);
+/* in_use: device memory of resident models and admitted requests; cycles and
+ * node_sweeps: 3D sweep cycles issued and the source node-sweeps they carried. */
+int pfast_server_info(pfast_server *srv, size_t *in_use, int *queued, int *running,
+                      long *cycles, double *node_sweeps);
+
 #ifdef __cplusplus
 }
 #endif

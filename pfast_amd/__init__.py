@@ -9,8 +9,10 @@ HIP kernels of libpfast.so through its C ABI (include/pfast.h):
     Tadjoint_fsm2d   <- src/RT_padjoint2d.h:288 (and the WR overload :67)
     Radjoint_fsm2d   <- src/RT_padjoint2d.h:382 (and the WR overload :159)
 
-plus the batched device-resident shot-set API (Context / Model3D / Shots3D) that
-replaces the drivers' per-source loops and MPI reductions.
+plus the batched device-resident shot-set API (Context / Model3D / Shots3D,
+Model2D / Shots2D) that replaces the drivers' per-source loops and MPI
+reductions, and a request Server (resident models, traveltime-table and gradient
+requests admitted by device-memory footprint and batched into shared sweeps).
 
 Arrays use the reference layouts: 3D fields are numpy arrays of shape
 (nz, nrows, ncols), 2D fields (nrows, ncols); src/rec are int32 index tables
@@ -21,13 +23,13 @@ import weakref
 
 import numpy as np
 
-from ._lib import (PFAST_FWD_ADJ, PFAST_FWD_ONLY, FatalError, PfastError, check, lib,  # noqa: F401
-                   ptr)
+from ._lib import (PFAST_FWD_ADJ, PFAST_FWD_ONLY, PFAST_REQ_GRADIENT, PFAST_REQ_TABLE,  # noqa: F401
+                   REQ_STATES, FatalError, PfastError, check, lib, ptr)
 
 INF = 9999999.0
 
 __all__ = ["fsm3d", "adjoint_fsm3d", "RT_fsm2d", "Tadjoint_fsm2d", "Radjoint_fsm2d", "sweep_variants",
-           "Context", "Model3D", "Shots3D", "Model2D", "Shots2D", "device_count", "PfastError", "FatalError", "INF"]
+           "Context", "Model3D", "Shots3D", "Model2D", "Shots2D", "Server", "device_count", "PfastError", "FatalError", "INF"]
 
 
 def _f64(a):
@@ -393,6 +395,149 @@ class Shots2D:
     def close(self):
         if self.h:
             lib().pfast_shots2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server on one GPU (pfast_server_* of include/pfast.h).
+
+    Models are added once and stay resident; submit3d / submit2d queue a
+    traveltime-table ("table") or gradient ("gradient") request for a source set
+    on a resident model and return a ticket; progress() admits, launches and
+    retires work.  Results stay on the device until release().
+    """
+
+    def __init__(self, device=0, budget_bytes=64 << 30):
+        h = C.c_void_p()
+        check(lib().pfast_server_create(int(device), C.c_size_t(int(budget_bytes)), C.byref(h)))
+        self.h = h
+        self._models = {}   # id -> (dim, shape, NumofR)
+        self._tickets = {}  # ticket -> (model id, nsrc)
+
+    def add_model3d(self, vp, dx, dy, dz):
+        vp = _f64(vp)
+        nz, nr, nc = vp.shape
+        mid = C.c_int(0)
+        check(lib().pfast_server_add_model3d(self.h, vp.ravel(), nr, nc, nz, float(dx), float(dy),
+                                             float(dz), C.byref(mid)))
+        self._models[mid.value] = (3, vp.shape, 0)
+        return mid.value
+
+    def add_model2d(self, vp, dx, dz, Z=None):
+        vp = _f64(vp)
+        nr, nc = vp.shape
+        R = 0 if Z is None else int(np.asarray(Z).shape[0])
+        Zf = None if Z is None else _f64(Z).ravel()
+        mid = C.c_int(0)
+        check(lib().pfast_server_add_model2d(self.h, vp.ravel(), nr, nc, float(dx), float(dz),
+                                             ptr(Zf), R, C.byref(mid)))
+        self._models[mid.value] = (2, vp.shape, R)
+        return mid.value
+
+    def remove_model(self, model):
+        check(lib().pfast_server_remove_model(self.h, int(model)))
+        del self._models[model]
+
+    @staticmethod
+    def _kind(kind):
+        return {"table": PFAST_REQ_TABLE, "gradient": PFAST_REQ_GRADIENT}[kind]
+
+    def submit3d(self, model, kind, src, srnum, rec=None, nxyz=None, T_obs=None, MAXSwp=100):
+        src = _i32(src).reshape(-1, 3)
+        srnum = _i32(srnum)
+        rec = None if rec is None else _i32(rec).reshape(-1, 3)
+        nxyz = None if nxyz is None else _f64(nxyz).reshape(-1, 3)
+        T_obs = None if T_obs is None else _f64(T_obs)
+        t = C.c_long(0)
+        check(lib().pfast_server_submit3d(self.h, int(model), self._kind(kind), src.shape[0],
+                                          src.ravel(), srnum, ptr(rec), ptr(nxyz), ptr(T_obs),
+                                          int(MAXSwp), C.byref(t)))
+        self._tickets[t.value] = (model, src.shape[0])
+        return t.value
+
+    def submit2d(self, model, kind, src, srnum, rec=None, nxy=None, TT_obs=None, TR_obs=None,
+                 MAXSwp=500):
+        src = _i32(src).reshape(-1, 2)
+        srnum = _i32(srnum)
+        rec = None if rec is None else _i32(rec).reshape(-1, 2)
+        nxy = None if nxy is None else _f64(nxy).reshape(-1, 2)
+        TT_obs = None if TT_obs is None else _f64(TT_obs)
+        TR_obs = None if TR_obs is None else _f64(TR_obs)
+        t = C.c_long(0)
+        check(lib().pfast_server_submit2d(self.h, int(model), self._kind(kind), src.shape[0],
+                                          src.ravel(), srnum, ptr(rec), ptr(nxy), ptr(TT_obs),
+                                          ptr(TR_obs), int(MAXSwp), C.byref(t)))
+        self._tickets[t.value] = (model, src.shape[0])
+        return t.value
+
+    def progress(self, wait_us=0):
+        n = C.c_int(0)
+        check(lib().pfast_server_progress(self.h, int(wait_us), C.byref(n)))
+        return n.value
+
+    def status(self, ticket):
+        """(state name, (t_submit, t_admit, t_done) in seconds since creation)."""
+        st = C.c_int(0)
+        t = np.zeros(3)
+        check(lib().pfast_server_status(self.h, int(ticket), C.byref(st), ptr(t)))
+        return REQ_STATES[st.value], tuple(t.tolist())
+
+    def error(self, ticket):
+        st, _ = self.status(ticket)
+        return lib().pfast_last_error().decode(errors="replace") if st == "failed" else None
+
+    def cycles(self, ticket):
+        model, nsrc = self._tickets[ticket]
+        dim, _, R = self._models[model]
+        c = np.zeros(2 * nsrc if dim == 3 else nsrc * (2 + 3 * R), np.int32)
+        check(lib().pfast_server_cycles(self.h, int(ticket), ptr(c)))
+        return c
+
+    def gradient(self, ticket):
+        """3D: (grad, misfit); 2D: (gT, gR, Et, Er)."""
+        model, _ = self._tickets[ticket]
+        dim, shape, _ = self._models[model]
+        g = np.empty(shape)
+        m = np.zeros(2)
+        if dim == 3:
+            check(lib().pfast_server_gradient(self.h, int(ticket), ptr(g), None, ptr(m)))
+            return g, float(m[0])
+        gr = np.empty(shape)
+        check(lib().pfast_server_gradient(self.h, int(ticket), ptr(g), ptr(gr), ptr(m)))
+        return g, gr, float(m[0]), float(m[1])
+
+    def field(self, ticket, s):
+        model, _ = self._tickets[ticket]
+        out = np.empty(self._models[model][1])
+        check(lib().pfast_server_field(self.h, int(ticket), int(s), out.ravel()))
+        return out
+
+    def footprint(self, ticket):
+        b = C.c_size_t(0)
+        check(lib().pfast_server_footprint(self.h, int(ticket), C.byref(b)))
+        return b.value
+
+    def release(self, ticket):
+        check(lib().pfast_server_release(self.h, int(ticket)))
+        self._tickets.pop(ticket, None)
+
+    def info(self):
+        used, q, r, cyc = C.c_size_t(0), C.c_int(0), C.c_int(0), C.c_long(0)
+        ns = C.c_double(0)
+        check(lib().pfast_server_info(self.h, C.byref(used), C.byref(q), C.byref(r), C.byref(cyc),
+                                      C.byref(ns)))
+        return dict(in_use=used.value, queued=q.value, running=r.value, cycles=cyc.value,
+                    node_sweeps=ns.value)
+
+    def close(self):
+        if self.h:
+            lib().pfast_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -18,6 +18,9 @@ PFAST_ERR_NOMEM = -3
 PFAST_ERR_FATAL = -4
 PFAST_FWD_ONLY = 1
 PFAST_FWD_ADJ = 2
+PFAST_REQ_TABLE = 1
+PFAST_REQ_GRADIENT = 2
+REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 
 class PfastError(RuntimeError):
@@ -74,6 +77,21 @@ _SIGS = {
     "pfast_shots2d_info": [vpp, vpp, vpp],
     "pfast_regu3d": [dp, dp, d, d, d, d, d, d, i, i, i],
     "pfast_regu2d": [dp, dp, d, d, d, d, i, i],
+    "pfast_server_create": [i, C.c_size_t, C.POINTER(vpp)],
+    "pfast_server_destroy": [vpp],
+    "pfast_server_add_model3d": [vpp, dp, i, i, i, d, d, d, pint],
+    "pfast_server_add_model2d": [vpp, dp, i, i, d, d, vpp, i, pint],
+    "pfast_server_remove_model": [vpp, i],
+    "pfast_server_submit3d": [vpp, i, i, i, ip, ip, vpp, vpp, vpp, i, plong],
+    "pfast_server_submit2d": [vpp, i, i, i, ip, ip, vpp, vpp, vpp, vpp, i, plong],
+    "pfast_server_progress": [vpp, i, pint],
+    "pfast_server_status": [vpp, C.c_long, pint, vpp],
+    "pfast_server_cycles": [vpp, C.c_long, vpp],
+    "pfast_server_gradient": [vpp, C.c_long, vpp, vpp, vpp],
+    "pfast_server_field": [vpp, C.c_long, i, dp],
+    "pfast_server_release": [vpp, C.c_long],
+    "pfast_server_footprint": [vpp, C.c_long, C.POINTER(C.c_size_t)],
+    "pfast_server_info": [vpp, C.POINTER(C.c_size_t), pint, pint, plong, pdbl],
 }
 # entry points returning double rather than a status code
 _DSIGS = {

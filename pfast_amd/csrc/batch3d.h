@@ -5,12 +5,19 @@
 // Sources per chunk (innermost dimension of the batched field layout).
 #define PFAST_CHUNK 4
 
+// One chunk of C sources as the sweep kernels see it.  A launch covers chunks
+// through a table of these, so the chunks of one launch may belong to different
+// shot sets and models of the same grid shape.
+struct ChunkRef {
+    float *field[2];           // [N][C]: [0] traveltimes (negative marks receiver
+                               // nodes during the adjoint), [1] lambda
+    const float *slow;         // [N] the model's cell time h/v (uniform) or slowness 1/v
+    int srcnode[PFAST_CHUNK];  // linear source node, -1 for padding sources
+};
+
 struct SlabArgs {
-    float *field;         // [nchunk][N][C]: traveltimes (forward) or lambda (adjoint)
-    const float *tfield;  // adjoint only: traveltimes (negative marks receiver nodes)
-    const float *slow;    // [N] cell time h/v (uniform spacing) or slowness 1/v
-    const int *srcnode;   // [nchunk*C] linear source node, -1 for padding sources
-    const int *active;    // chunk ids still sweeping
+    const ChunkRef *chunks;  // indexed by chunk id
+    const int *active;       // chunk ids still sweeping
     unsigned *changed;    // [nchunk] bit c: source c of the chunk changed this cycle
     int nact, nact_p8;    // active chunks; grid stride (multiple of 8 = XCD count)
     int nrows, ncols, nz, nti, ntj;

@@ -1,9 +1,10 @@
 // batch3d.hip -- batched FP32 3D sweeps (every source of a shot set on one GPU).
 //
 // Layout in HBM: fields are stored per chunk of C sources, node-major with the C
-// sources innermost: F[chunk][k][i][j][c] (k = depth, i = row, j = column).  A
-// velocity-derived cell time f[k][i][j] (h/v for uniform spacing, else 1/v) is
-// shared by every chunk.
+// sources innermost: F[k][i][j][c] (k = depth, i = row, j = column).  Each chunk
+// names its traveltime and lambda fields and its model's velocity-derived cell
+// time f[k][i][j] (h/v for uniform spacing, else 1/v) through a ChunkRef, so one
+// launch can sweep chunks of several shot sets and models of one grid shape.
 //
 // Schedule ("slab ring"): the (i,j) plane is cut into TI x TJ column tiles that
 // run the whole depth.  For one ordering (s3,s1,s2) tile (ti',tj') (coordinates
@@ -178,8 +179,8 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
-    float *__restrict__ Fc = a.field + (size_t)t.chunk * plane * nz * C;
-    const float *__restrict__ slow = a.slow;
+    float *__restrict__ Fc = a.chunks[t.chunk].field[0];
+    const float *__restrict__ slow = a.chunks[t.chunk].slow;
     // cell times: threads 0..FS-1 stream one node each
     const bool fown = tid < FS && (tid / TJ) < t.TIe && (tid % TJ) < t.TJe;
     const unsigned org = (unsigned)t.i0 * a.ncols + t.j0;
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s
     // level (in sweep order) at which this thread's column holds its own source
     int srclev = -1;
     {
-        const int sn = a.srcnode[t.chunk * C + l.c];
+        const int sn = a.chunks[t.chunk].srcnode[l.c];
         if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
             const int sk = (int)((unsigned)sn / plane);
             srclev = s3 > 0 ? sk : nz - 1 - sk;
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s
 }
 
 // ------------------------------------------------------------------- adjoint
-// lambda in a.field, traveltimes in a.tfield (read-only; a negative value marks a
+// lambda in field[1] of each chunk, traveltimes in field[0] (read-only; a negative value marks a
 // receiver node of that source, whose lambda is fixed by the boundary condition).
 // Only interior nodes are updated, so no edge sentinels are needed.
 template <int TI, int TJ, int C, int L>
@@ -301,9 +302,8 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_slab(SlabArgs a, int D, int s
     const Lane<TI, TJ, C> l = make_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const size_t cb = (size_t)t.chunk * plane * nz * C;
-    float *__restrict__ Lm = a.field + cb;
-    const float *__restrict__ Tm = a.tfield + cb;
+    float *__restrict__ Lm = a.chunks[t.chunk].field[1];
+    const float *__restrict__ Tm = a.chunks[t.chunk].field[0];
     const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
     const bool col_interior =
         l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
@@ -523,8 +523,8 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
     const unsigned fbytes = 4u * plane * (unsigned)nz * C;
-    const rsrc_t RF = make_rsrc(a.field + (size_t)t.chunk * plane * nz * C, fbytes);
-    const rsrc_t RS = make_rsrc(a.slow, 4u * plane * (unsigned)nz);
+    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
+    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
     const float INF = PFAST_INF_F;
     if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF;
     // steps p at which this thread updates its node: pipj <= p < pipj + nz, except
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
     int psrc = -1;
     {
-        const int sn = a.srcnode[t.chunk * C + l.c];
+        const int sn = a.chunks[t.chunk].srcnode[l.c];
         if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
             const int sk = (int)((unsigned)sn / plane);
             psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
@@ -643,10 +643,9 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const size_t cb = (size_t)t.chunk * plane * nz * C;
     const unsigned fbytes = 4u * plane * (unsigned)nz * C;
-    const rsrc_t RL = make_rsrc(a.field + cb, fbytes);
-    const rsrc_t RT = make_rsrc(a.tfield + cb, fbytes);
+    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
+    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
     const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
     const bool col_interior =
         l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
@@ -835,8 +834,8 @@ __global__ __launch_bounds__(TI *TJ *C + 64) void fwd3d_dma(SlabArgs a, int D, i
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
-    const rsrc_t RF = make_rsrc(a.field + (size_t)t.chunk * plane * nz * C, 4u * pC * nz);
-    const rsrc_t RS = make_rsrc(a.slow, 4u * plane * nz);
+    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], 4u * pC * nz);
+    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * nz);
     const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
     const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
     const unsigned stF = s3 > 0 ? 4u * plane : 0u - 4u * plane;
@@ -895,7 +894,7 @@ __global__ __launch_bounds__(TI *TJ *C + 64) void fwd3d_dma(SlabArgs a, int D, i
     const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
     int psrc = -1;
     {
-        const int sn = a.srcnode[t.chunk * C + l.c];
+        const int sn = a.chunks[t.chunk].srcnode[l.c];
         if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
             const int sk = (int)((unsigned)sn / plane);
             psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
@@ -965,9 +964,8 @@ __global__ __launch_bounds__(TI *TJ *C + 64) void adj3d_dma(SlabArgs a, int D, i
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
-    const size_t cb = (size_t)t.chunk * plane * nz * C;
-    const rsrc_t RL = make_rsrc(a.field + cb, 4u * pC * nz);
-    const rsrc_t RT = make_rsrc(a.tfield + cb, 4u * pC * nz);
+    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], 4u * pC * nz);
+    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], 4u * pC * nz);
     const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
     const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
     const unsigned NONE = 0x80000000u;

@@ -16,6 +16,7 @@
 #include "context.h"
 #include "pfast_internal.h"
 #include "plain_kernels.h"
+#include "shots3d.h"
 
 namespace pfast {
 static thread_local std::string g_err;
@@ -538,49 +539,6 @@ int pfast_radjoint_fsm2d(double *UR, const double *Td, const double *Tu, const d
 }  // extern "C"
 
 // ======================================================= batched 3D
-struct pfast_model3d {
-    pfast_ctx *ctx;
-    int nrows, ncols, nz;
-    double dx, dy, dz;
-    bool uniform;
-    double *vp = nullptr;  // N doubles
-    float *f = nullptr;    // N cell times / slowness
-    int nshots = 0;        // live shot sets bound to this model
-};
-
-struct pfast_shots3d {
-    pfast_model3d *m;
-    int nsrc, nchunk;
-    long nrec;
-    std::vector<int> src;    // 3*nsrc
-    std::vector<int> srnum;  // nsrc
-    std::vector<long> ch0;   // nsrc+1 prefix
-    int *srcnode = nullptr;  // [nchunk*C]
-    float *T = nullptr, *L = nullptr;
-    // receiver table (reference order) for picks and the single-source path
-    long *rnode = nullptr;
-    int *rsrc = nullptr;
-    int *rec3 = nullptr;
-    double *nxyz = nullptr, *Tobs = nullptr;
-    // picks grouped by source (misfit, fixed-node marks)
-    long *pnode = nullptr;
-    int *psrc = nullptr;
-    double *pobs = nullptr;
-    long *poff = nullptr;
-    int npick = 0;
-    // boundary-condition entries (last picked receiver per node, not at the source)
-    long *bnode = nullptr;
-    int *bsrc = nullptr;
-    double *bobs = nullptr, *bnrm = nullptr;
-    int nb = 0;
-    double *EE = nullptr, *grad = nullptr, *picks = nullptr, *fieldbuf = nullptr;
-    unsigned *changed = nullptr;
-    int *active = nullptr;
-    unsigned *hchanged = nullptr;
-    std::vector<int> fcyc, acyc;
-    bool have_adj_data = false;
-};
-
 namespace {
 
 // Sweep cycles of the batched FP32 solver until every source is stationary.
@@ -589,27 +547,12 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
     pfast_model3d *m = s->m;
     pfast_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
-    const SweepVariant &sv = pfast::sweep_variant(ctx->sweep);
-    const int TI = sv.ti, TJ = sv.tj;
-    const int nti = (m->nrows + TI - 1) / TI, ntj = (m->ncols + TJ - 1) / TJ;
-    SlabArgs a{};
-    a.field = adjoint ? s->L : s->T;
-    a.tfield = s->T;
-    a.slow = m->f;
-    a.srcnode = s->srcnode;
-    a.active = s->active;
-    a.changed = s->changed;
-    a.nrows = m->nrows;
-    a.ncols = m->ncols;
-    a.nz = m->nz;
-    a.nti = nti;
-    a.ntj = ntj;
-    a.hx = (float)m->dx;
-    a.hy = (float)m->dy;
-    a.hz = (float)m->dz;
-    a.wx = (float)(1.0 / (m->dx * m->dx));
-    a.wy = (float)(1.0 / (m->dy * m->dy));
-    a.wz = (float)(1.0 / (m->dz * m->dz));
+    const pfast::Grid3 g = pfast::grid_of(m);
+    if (!s->hchanged && hipHostMalloc((void **)&s->hchanged, s->nchunk * sizeof(unsigned)) != hipSuccess) {
+        s->hchanged = nullptr;
+        pfast::set_error("hipHostMalloc failed");
+        return PFAST_ERR_NOMEM;
+    }
     std::vector<int> act(s->nchunk);
     for (int i = 0; i < s->nchunk; i++) act[i] = i;
     cyc.assign(s->nsrc, 0);
@@ -618,21 +561,13 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
         cycle++;
         HIPCHK(hipMemcpyAsync(s->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
-        a.nact = (int)act.size();
-        a.nact_p8 = (a.nact + 7) / 8 * 8;
+        const int n = (int)act.size();
         long real = 0;
         for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
         ctx->units[adjoint ? 1 : 0] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
-        for (int o = 0; o < 8; o++) {
-            const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
-            for (int D = 0; D <= nti + ntj - 2; D++) {
-                const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
-                pfast::time_begin(ctx, adjoint ? 1 : 0);
-                HIPCHK(pfast::launch_sweep3d(st, a, ctx->sweep, adjoint, m->uniform, D, s1, s2, s3,
-                                              hi - lo + 1));
-                pfast::time_end(ctx);
-            }
-        }
+        int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, s->active, n, s->changed)
+                         : pfast::issue_cycle3d(ctx, g, s->chunks, s->active, n, nullptr, 0, s->changed);
+        if (rc) return rc;
         HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         pfast::time_collect(ctx);
@@ -653,6 +588,143 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
 }
 
 }  // namespace
+
+namespace pfast {
+
+int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                  int nfwd, const int *adj, int nadj, unsigned *changed)
+{
+    hipStream_t st = ctx->stream;
+    const SweepVariant &sv = sweep_variant(ctx->sweep);
+    const int nti = (g.nrows + sv.ti - 1) / sv.ti, ntj = (g.ncols + sv.tj - 1) / sv.tj;
+    SlabArgs a{};
+    a.chunks = chunks;
+    a.changed = changed;
+    a.nrows = g.nrows;
+    a.ncols = g.ncols;
+    a.nz = g.nz;
+    a.nti = nti;
+    a.ntj = ntj;
+    a.hx = (float)g.dx;
+    a.hy = (float)g.dy;
+    a.hz = (float)g.dz;
+    a.wx = (float)(1.0 / (g.dx * g.dx));
+    a.wy = (float)(1.0 / (g.dy * g.dy));
+    a.wz = (float)(1.0 / (g.dz * g.dz));
+    SlabArgs af = a, aa = a;
+    af.active = fwd;
+    af.nact = nfwd;
+    af.nact_p8 = (nfwd + 7) / 8 * 8;
+    aa.active = adj;
+    aa.nact = nadj;
+    aa.nact_p8 = (nadj + 7) / 8 * 8;
+    for (int o = 0; o < 8; o++) {
+        const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+        for (int D = 0; D <= nti + ntj - 2; D++) {
+            const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
+            if (nfwd) {
+                time_begin(ctx, 0);
+                HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
+                time_end(ctx);
+            }
+            if (nadj) {
+                time_begin(ctx, 1);
+                HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
+                time_end(ctx);
+            }
+        }
+    }
+    return PFAST_OK;
+}
+
+int fwd_begin3d(pfast_shots3d *s)
+{
+    const long N = (long)s->m->nrows * s->m->ncols * s->m->nz;
+    HIPCHK(launch_init_fwd(s->m->ctx->stream, s->T, s->srcnode, N, s->nchunk));
+    return PFAST_OK;
+}
+
+int adj_begin3d(pfast_shots3d *s)
+{
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    HIPCHK(launch_adj_misfit(st, s->T, s->pnode, s->pobs, s->poff, s->nsrc, N, s->EE));
+    HIPCHK(hipMemsetAsync(s->L, 0, (size_t)s->nchunk * N * PFAST_CHUNK * sizeof(float), st));
+    HIPCHK(launch_adj_bc(st, s->L, s->T, s->bnode, s->bsrc, s->bobs, s->bnrm, s->nb, m->nrows,
+                         m->ncols, m->nz, m->dx, m->dy, m->dz));
+    HIPCHK(launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 0));
+    return PFAST_OK;
+}
+
+int adj_end3d(pfast_shots3d *s)
+{
+    pfast_model3d *m = s->m;
+    hipStream_t st = m->ctx->stream;
+    const long N = (long)m->nrows * m->ncols * m->nz;
+    HIPCHK(launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 1));
+    HIPCHK(launch_grad_reduce(st, s->L, m->vp, s->grad, N, s->nchunk, s->nsrc));
+    return PFAST_OK;
+}
+
+size_t pool_round(size_t bytes)
+{
+    if (bytes <= (size_t(1) << 20)) {
+        size_t r = 256;
+        while (r < bytes) r <<= 1;
+        return r;
+    }
+    const size_t g = size_t(2) << 20;
+    return (bytes + g - 1) / g * g;
+}
+
+void *pool_get(DevicePool *p, size_t bytes)
+{
+    const size_t b = pool_round(bytes);
+    auto it = p->freel.find(b);
+    if (it != p->freel.end() && !it->second.empty()) {
+        void *q = it->second.back();
+        it->second.pop_back();
+        p->cached -= b;
+        return q;
+    }
+    void *q = nullptr;
+    if (hipMalloc(&q, b) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim(p, 0);
+        if (hipMalloc(&q, b) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    p->sizes[q] = b;
+    return q;
+}
+
+void pool_put(DevicePool *p, void *q)
+{
+    auto it = p->sizes.find(q);
+    if (it == p->sizes.end()) {
+        (void)hipFree(q);
+        return;
+    }
+    p->freel[it->second].push_back(q);
+    p->cached += it->second;
+}
+
+void pool_trim(DevicePool *p, size_t keep)
+{
+    for (auto it = p->freel.rbegin(); it != p->freel.rend() && p->cached > keep; ++it)
+        while (!it->second.empty() && p->cached > keep) {
+            void *q = it->second.back();
+            it->second.pop_back();
+            (void)hipFree(q);
+            p->sizes.erase(q);
+            p->cached -= it->first;
+        }
+}
+
+}  // namespace pfast
 
 extern "C" {
 
@@ -675,7 +747,7 @@ int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     m->uniform = dx == dy && dy == dz;
     const long N = (long)nrows * ncols * nz;
     int rc;
-    if ((rc = pfast::dalloc(&m->vp, N)) || (rc = pfast::dalloc(&m->f, N))) {
+    if ((rc = pfast::dalloc(ctx, &m->vp, N)) || (rc = pfast::dalloc(ctx, &m->f, N))) {
         pfast_model3d_destroy(m);
         return rc;
     }
@@ -705,8 +777,8 @@ int pfast_model3d_destroy(pfast_model3d *m)
     ARGCHK(m->nshots == 0, "model still has live shot sets: destroy them first");
     (void)hipStreamSynchronize(m->ctx->stream);
     m->ctx->nmodels--;
-    if (m->vp) (void)hipFree(m->vp);
-    if (m->f) (void)hipFree(m->f);
+    pfast::dfree(m->ctx, m->vp);
+    pfast::dfree(m->ctx, m->f);
     delete m;
     return PFAST_OK;
 }
@@ -715,16 +787,28 @@ int pfast_shots3d_destroy(pfast_shots3d *s)
 {
     if (!s) return PFAST_OK;
     (void)hipStreamSynchronize(s->m->ctx->stream);
+    pfast::shots3d_free(s);
+    return PFAST_OK;
+}
+
+}  // extern "C"
+
+namespace pfast {
+
+void shots3d_free(pfast_shots3d *s)
+{
     s->m->nshots--;
     void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
                     s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
-                    s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+                    s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active, s->chunks};
+    for (void *p : ptrs) pfast::dfree(s->m->ctx, p);
     if (s->hchanged) (void)hipHostFree(s->hchanged);
     delete s;
-    return PFAST_OK;
 }
+
+}  // namespace pfast
+
+extern "C" {
 
 int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
                          const int *rec, const double *nxyz, const double *T_obs,
@@ -770,17 +854,12 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         sn[i] = (int)(((long)src[3 * i + 2] * nrows + src[3 * i]) * ncols + src[3 * i + 1]);
     int rc = 0;
     const size_t fsz = (size_t)s->nchunk * N * PFAST_CHUNK;
-    if ((rc = pfast::dupload(&s->srcnode, sn.data(), sn.size(), st)) || (rc = pfast::dalloc(&s->T, fsz)) ||
-        (rc = pfast::dalloc(&s->changed, s->nchunk)) || (rc = pfast::dalloc(&s->active, s->nchunk)) ||
-        (rc = pfast::dalloc(&s->EE, nsrc)) || (rc = pfast::dalloc(&s->grad, N)) ||
-        (rc = pfast::dalloc(&s->fieldbuf, N))) {
+    if ((rc = pfast::dupload(m->ctx, &s->srcnode, sn.data(), sn.size())) || (rc = pfast::dalloc(m->ctx, &s->T, fsz)) ||
+        (rc = pfast::dalloc(m->ctx, &s->changed, s->nchunk)) || (rc = pfast::dalloc(m->ctx, &s->active, s->nchunk)) ||
+        (rc = pfast::dalloc(m->ctx, &s->EE, nsrc)) || (rc = pfast::dalloc(m->ctx, &s->grad, N)) ||
+        (rc = pfast::dalloc(m->ctx, &s->fieldbuf, N))) {
         pfast_shots3d_destroy(s);
         return rc;
-    }
-    if (hipHostMalloc((void **)&s->hchanged, s->nchunk * sizeof(unsigned)) != hipSuccess) {
-        pfast_shots3d_destroy(s);
-        pfast::set_error("hipHostMalloc failed");
-        return PFAST_ERR_NOMEM;
     }
     // receiver table: node index and owning source of each entry
     std::vector<long> rnode(s->nrec);
@@ -792,9 +871,9 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
                 rnode[r] = ((long)q[2] * nrows + q[0]) * ncols + q[1];
                 rsrc[r] = i;
             }
-        if ((rc = pfast::dupload(&s->rnode, rnode.data(), s->nrec, st)) ||
-            (rc = pfast::dupload(&s->rsrc, rsrc.data(), s->nrec, st)) ||
-            (rc = pfast::dupload(&s->rec3, rec, 3 * s->nrec, st)) || (rc = pfast::dalloc(&s->picks, s->nrec))) {
+        if ((rc = pfast::dupload(m->ctx, &s->rnode, rnode.data(), s->nrec)) ||
+            (rc = pfast::dupload(m->ctx, &s->rsrc, rsrc.data(), s->nrec)) ||
+            (rc = pfast::dupload(m->ctx, &s->rec3, rec, 3 * s->nrec)) || (rc = pfast::dalloc(m->ctx, &s->picks, s->nrec))) {
             pfast_shots3d_destroy(s);
             return rc;
         }
@@ -827,19 +906,31 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         }
         s->npick = (int)pnode.size();
         s->nb = (int)bnode.size();
-        if ((rc = pfast::dalloc(&s->L, fsz)) || (rc = pfast::dupload(&s->pnode, pnode.data(), pnode.size(), st)) ||
-            (rc = pfast::dupload(&s->psrc, psrc.data(), psrc.size(), st)) ||
-            (rc = pfast::dupload(&s->pobs, pobs.data(), pobs.size(), st)) ||
-            (rc = pfast::dupload(&s->poff, poff.data(), poff.size(), st)) ||
-            (rc = pfast::dupload(&s->bnode, bnode.data(), bnode.size(), st)) ||
-            (rc = pfast::dupload(&s->bsrc, bsrc.data(), bsrc.size(), st)) ||
-            (rc = pfast::dupload(&s->bobs, bobs.data(), bobs.size(), st)) ||
-            (rc = pfast::dupload(&s->bnrm, bnrm.data(), bnrm.size(), st)) ||
-            (rc = pfast::dupload(&s->nxyz, nxyz, 3 * s->nrec, st)) ||
-            (rc = pfast::dupload(&s->Tobs, T_obs, s->nrec, st))) {
+        if ((rc = pfast::dalloc(m->ctx, &s->L, fsz)) || (rc = pfast::dupload(m->ctx, &s->pnode, pnode.data(), pnode.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->psrc, psrc.data(), psrc.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->pobs, pobs.data(), pobs.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->poff, poff.data(), poff.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->bnode, bnode.data(), bnode.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->bsrc, bsrc.data(), bsrc.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->bobs, bobs.data(), bobs.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->bnrm, bnrm.data(), bnrm.size())) ||
+            (rc = pfast::dupload(m->ctx, &s->nxyz, nxyz, 3 * s->nrec)) ||
+            (rc = pfast::dupload(m->ctx, &s->Tobs, T_obs, s->nrec))) {
             pfast_shots3d_destroy(s);
             return rc;
         }
+    }
+    s->hchunks.resize(s->nchunk);
+    for (int i = 0; i < s->nchunk; i++) {
+        ChunkRef &c = s->hchunks[i];
+        c.field[0] = s->T + (size_t)i * N * PFAST_CHUNK;
+        c.field[1] = s->L ? s->L + (size_t)i * N * PFAST_CHUNK : nullptr;
+        c.slow = m->f;
+        for (int k = 0; k < PFAST_CHUNK; k++) c.srcnode[k] = sn[i * PFAST_CHUNK + k];
+    }
+    if ((rc = pfast::dupload(m->ctx, &s->chunks, s->hchunks.data(), s->hchunks.size()))) {
+        pfast_shots3d_destroy(s);
+        return rc;
     }
     HIPCHK(hipStreamSynchronize(st));
     *out = s;
@@ -855,20 +946,12 @@ int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp)
     pfast_model3d *m = s->m;
     hipStream_t st = m->ctx->stream;
     HIPCHK(hipSetDevice(m->ctx->device));
-    const long N = (long)m->nrows * m->ncols * m->nz;
-    HIPCHK(pfast::launch_init_fwd(st, s->T, s->srcnode, N, s->nchunk));
-    int rc = run_sweeps3d(s, false, MAXSwp, s->fcyc);
-    if (rc) return rc;
-    if (mode == PFAST_FWD_ONLY) return PFAST_OK;
-    HIPCHK(pfast::launch_adj_misfit(st, s->T, s->pnode, s->pobs, s->poff, s->nsrc, N, s->EE));
-    HIPCHK(hipMemsetAsync(s->L, 0, (size_t)s->nchunk * N * PFAST_CHUNK * sizeof(float), st));
-    HIPCHK(pfast::launch_adj_bc(st, s->L, s->T, s->bnode, s->bsrc, s->bobs, s->bnrm, s->nb,
-                                m->nrows, m->ncols, m->nz, m->dx, m->dy, m->dz));
-    HIPCHK(pfast::launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 0));
-    rc = run_sweeps3d(s, true, MAXSwp, s->acyc);
-    if (rc) return rc;
-    HIPCHK(pfast::launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 1));
-    HIPCHK(pfast::launch_grad_reduce(st, s->L, m->vp, s->grad, N, s->nchunk, s->nsrc));
+    int rc = pfast::fwd_begin3d(s);
+    if (!rc) rc = run_sweeps3d(s, false, MAXSwp, s->fcyc);
+    if (rc || mode == PFAST_FWD_ONLY) return rc;
+    if ((rc = pfast::adj_begin3d(s)) || (rc = run_sweeps3d(s, true, MAXSwp, s->acyc)) ||
+        (rc = pfast::adj_end3d(s)))
+        return rc;
     HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;
 }

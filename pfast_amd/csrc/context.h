@@ -12,6 +12,15 @@
 #include "../../include/pfast.h"
 #include "pfast_internal.h"
 
+// Caching device allocator of a request server: blocks freed by models and shot
+// sets return here instead of to hipFree (which synchronises the device) and are
+// handed out again for requests of the same size class.
+struct DevicePool {
+    std::map<size_t, std::vector<void *>> freel;  // size class -> cached blocks
+    std::map<void *, size_t> sizes;               // every block of the pool
+    size_t cached = 0;                            // bytes in freel
+};
+
 struct pfast_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -26,6 +35,7 @@ struct pfast_ctx {
     size_t evnext = 0;
     // grow-only device scratch for the single-source forms
     std::map<std::string, std::pair<void *, size_t>> cache;
+    DevicePool *pool = nullptr;  // set for the contexts of a request server
 };
 
 namespace pfast {
@@ -55,6 +65,45 @@ int dupload(T **p, const T *h, size_t count, hipStream_t st)
     if (rc) return rc;
     if (count && h) HIPCHK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
     return PFAST_OK;
+}
+
+// Pool size class: powers of two up to 1 MiB, then multiples of 2 MiB.
+size_t pool_round(size_t bytes);
+void *pool_get(DevicePool *p, size_t bytes);  // nullptr when the device is out of memory
+void pool_put(DevicePool *p, void *q);
+void pool_trim(DevicePool *p, size_t keep);   // hipFree cached blocks down to `keep` bytes
+
+// Allocation of context-owned objects (models, shot sets): the context's pool
+// when it has one, else hipMalloc / hipFree.
+template <class T>
+int dalloc(pfast_ctx *ctx, T **p, size_t count)
+{
+    if (!ctx || !ctx->pool) return dalloc(p, count);
+    *p = (T *)pool_get(ctx->pool, std::max<size_t>(count, 1) * sizeof(T));
+    if (!*p) {
+        set_error("device allocation failed (" + std::to_string(count * sizeof(T)) + " bytes)");
+        return PFAST_ERR_NOMEM;
+    }
+    return PFAST_OK;
+}
+
+template <class T>
+int dupload(pfast_ctx *ctx, T **p, const T *h, size_t count)
+{
+    int rc = dalloc(ctx, p, count);
+    if (rc) return rc;
+    if (count && h)
+        HIPCHK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return PFAST_OK;
+}
+
+inline void dfree(pfast_ctx *ctx, void *p)
+{
+    if (!p) return;
+    if (ctx && ctx->pool)
+        pool_put(ctx->pool, p);
+    else
+        (void)hipFree(p);
 }
 
 }  // namespace pfast

@@ -82,13 +82,17 @@ int regu_common(double *Dj, const double *vp, double nux, double nuy, double nuz
         fail("hipMalloc failed in regularisation", PFAST_ERR_NOMEM);
     }
     if (!rc) {
-        hipMemcpy(dD, Dj, N * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(dvp, vp, N * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(dk, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(dk + kx.size(), ky.data(), ky.size() * sizeof(double), hipMemcpyHostToDevice);
-        if (three)
-            hipMemcpy(dk + kx.size() + ky.size(), kz.data(), kz.size() * sizeof(double),
-                      hipMemcpyHostToDevice);
+        const bool ok =
+            hipMemcpy(dD, Dj, N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dvp, vp, N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dk, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(dk + kx.size(), ky.data(), ky.size() * sizeof(double), hipMemcpyHostToDevice) ==
+                hipSuccess &&
+            (!three || hipMemcpy(dk + kx.size() + ky.size(), kz.data(), kz.size() * sizeof(double),
+                                 hipMemcpyHostToDevice) == hipSuccess);
+        if (!ok) fail("hipMemcpy failed in regularisation", PFAST_ERR_HIP);
+    }
+    if (!rc) {
         hipfftResult fr = three ? hipfftPlan3d(&plan, nz, nrows, ncols, HIPFFT_Z2Z)
                                 : hipfftPlan2d(&plan, nrows, ncols, HIPFFT_Z2Z);
         if (fr != HIPFFT_SUCCESS) fail("hipfftPlan failed", PFAST_ERR_HIP);
@@ -96,20 +100,29 @@ int regu_common(double *Dj, const double *vp, double nux, double nuy, double nuz
     if (!rc) {
         const unsigned g = (unsigned)((N + 255) / 256);
         hipLaunchKernelGGL(pack_real, dim3(g), dim3(256), 0, 0, dD, F, N);
-        if (hipfftExecZ2Z(plan, F, F, HIPFFT_FORWARD) != HIPFFT_SUCCESS) fail("hipfftExec", PFAST_ERR_HIP);
+        if (hipGetLastError() != hipSuccess) fail("pack kernel launch failed", PFAST_ERR_HIP);
+    }
+    if (!rc && hipfftExecZ2Z(plan, F, F, HIPFFT_FORWARD) != HIPFFT_SUCCESS) fail("hipfftExec", PFAST_ERR_HIP);
+    if (!rc) {
+        const unsigned g = (unsigned)((N + 255) / 256);
         hipLaunchKernelGGL(smooth_filter, dim3(g), dim3(256), 0, 0, F, dk, dk + kx.size(),
                            three ? dk + kx.size() + ky.size() : nullptr, nux, three ? nuy : nuz,
                            nuz, nrows, ncols, N);
-        if (hipfftExecZ2Z(plan, F, F, HIPFFT_BACKWARD) != HIPFFT_SUCCESS) fail("hipfftExec", PFAST_ERR_HIP);
-        hipLaunchKernelGGL(unpack_mask, dim3(g), dim3(256), 0, 0, F, dvp, dD, N);
-        if (hipMemcpy(Dj, dD, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-            fail("hipMemcpy failed in regularisation", PFAST_ERR_HIP);
+        if (hipGetLastError() != hipSuccess) fail("filter kernel launch failed", PFAST_ERR_HIP);
     }
-    if (plan) hipfftDestroy(plan);
-    hipFree(dD);
-    hipFree(dvp);
-    hipFree(F);
-    hipFree(dk);
+    if (!rc && hipfftExecZ2Z(plan, F, F, HIPFFT_BACKWARD) != HIPFFT_SUCCESS) fail("hipfftExec", PFAST_ERR_HIP);
+    if (!rc) {
+        const unsigned g = (unsigned)((N + 255) / 256);
+        hipLaunchKernelGGL(unpack_mask, dim3(g), dim3(256), 0, 0, F, dvp, dD, N);
+        if (hipGetLastError() != hipSuccess) fail("unpack kernel launch failed", PFAST_ERR_HIP);
+    }
+    if (!rc && hipMemcpy(Dj, dD, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        fail("hipMemcpy failed in regularisation", PFAST_ERR_HIP);
+    if (plan) (void)hipfftDestroy(plan);
+    (void)hipFree(dD);
+    (void)hipFree(dvp);
+    (void)hipFree(F);
+    (void)hipFree(dk);
     return rc;
 }
 

@@ -17,32 +17,7 @@
 #include "context.h"
 #include "pfast_internal.h"
 #include "plain_kernels.h"
-
-struct pfast_model2d {
-    pfast_ctx *ctx;
-    int nrows, ncols, NumofR;
-    double dx, dz;
-    double *vp = nullptr;  // N doubles, reference layout vp[i * ncols + j]
-    double *Z = nullptr;   // NumofR * ncols * 3 (depth, n_z, n_x), reference layout
-    int nshots = 0;
-};
-
-struct pfast_shots2d {
-    pfast_model2d *m;
-    int nsrc;
-    long nrec;
-    std::vector<long> ch0;
-    int *src = nullptr, *rec = nullptr, *srnum = nullptr;
-    long *dch0 = nullptr;
-    double *nxy = nullptr, *TT_obs = nullptr, *TR_obs = nullptr;
-    float *TT = nullptr, *TTd = nullptr, *TTu = nullptr;  // [s][N], [s][R][N]
-    float *LT = nullptr, *LR = nullptr, *Uw = nullptr;    // adjoints, work field
-    unsigned char *tag = nullptr;
-    double *EEt = nullptr, *EEr = nullptr;
-    int *fcyc = nullptr, *tcyc = nullptr, *rcyc = nullptr, *status = nullptr, *k2 = nullptr;
-    double *gT = nullptr, *gR = nullptr, *gRT = nullptr, *WR = nullptr, *fieldbuf = nullptr;
-    bool have_fwd = false, have_adj = false;
-};
+#include "shots2d.h"
 
 namespace {
 
@@ -105,6 +80,83 @@ Grid2 grid_of(const pfast_model2d *m)
 
 }  // namespace
 
+namespace pfast {
+
+int issue_fwd2d(pfast_shots2d *s, int MAXSwp, int *status_host)
+{
+    pfast_model2d *m = s->m;
+    pfast_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const Grid2 g = grid_of(m);
+    s->have_fwd = s->have_adj = false;
+    time_begin(ctx, 2);
+    HIPCHK(launch_rt_fsm2d_f32(st, m->vp, s->TT, s->TTd, s->TTu, m->Z, s->src, s->nsrc, m->NumofR,
+                               MAXSwp, g, s->fcyc, s->status, s->k2));
+    time_end(ctx);
+    HIPCHK(hipMemcpyAsync(status_host, s->status, s->nsrc * sizeof(int), hipMemcpyDeviceToHost, st));
+    return PFAST_OK;
+}
+
+int check_fwd2d(pfast_shots2d *s, const int *status)
+{
+    for (int i = 0; i < s->nsrc; i++)
+        if (status[i]) {
+            set_error(std::string("source ") + std::to_string(i) +
+                      (status[i] == 1 ? ": no k2 above the reflector in a column (the "
+                                        "reference exits, src/RT_pfsm2d.h:232,257,281)"
+                                      : ": negative reflected traveltime seed (the "
+                                        "reference exits, src/RT_pfsm2d.h:227,253,274)"));
+            return PFAST_ERR_FATAL;
+        }
+    s->have_fwd = true;
+    return PFAST_OK;
+}
+
+int issue_adj2d(pfast_shots2d *s, int MAXSwp)
+{
+    pfast_model2d *m = s->m;
+    pfast_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const Grid2 g = grid_of(m);
+    const int R = m->NumofR;
+    time_begin(ctx, 2);
+    HIPCHK(launch_tadj2d_f32(st, s->LT, s->tag, s->TT, s->TT_obs, 1.0, s->rec, s->nxy, s->dch0,
+                             s->srnum, s->src, s->nsrc, MAXSwp, g, s->EEt, s->tcyc));
+    time_end(ctx);
+    if (R > 0) {
+        time_begin(ctx, 2);
+        HIPCHK(launch_radj2d_f32(st, s->LR, s->Uw, s->tag, s->TTu, s->TR_obs, 1.0, s->rec, s->nxy,
+                                 s->dch0, s->srnum, s->src, s->nsrc, R, MAXSwp, g, s->EEr, s->rcyc));
+        time_end(ctx);
+    } else {
+        HIPCHK(hipMemsetAsync(s->EEr, 0, s->nsrc * sizeof(double), st));
+    }
+    return PFAST_OK;
+}
+
+int issue_grad2d(pfast_shots2d *s, const double *WR_dev)
+{
+    pfast_model2d *m = s->m;
+    const long N = (long)m->nrows * m->ncols;
+    hipLaunchKernelGGL(grad2d_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       m->ctx->stream, s->LT, s->LR, m->vp, WR_dev, s->gT, s->gR, s->gRT, N,
+                       s->nsrc, m->NumofR > 0 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return PFAST_OK;
+}
+
+size_t footprint2d(int nrows, int ncols, int NumofR, int nsrc, long nrec, bool adjoint)
+{
+    const size_t N = (size_t)nrows * ncols, F = (size_t)nsrc * N, R1 = NumofR ? NumofR : 1;
+    size_t b = F * 4 * (1 + 2 * R1) + N * 8 + (size_t)nsrc * (4 * (4 + 2 * NumofR + ncols) + 8);
+    if (adjoint)
+        b += F * (4 * 3 + 1) + N * 8 * 4 + (size_t)nrec * (8 + 16 + 8 + 8 * NumofR) +
+             (size_t)nsrc * (8 * 2 + 4 * (1 + R1));
+    return b;
+}
+
+}  // namespace pfast
+
 extern "C" {
 
 int pfast_model2d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols, double dx,
@@ -124,8 +176,8 @@ int pfast_model2d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     m->dx = dx;
     m->dz = dz;
     const long N = (long)nrows * ncols;
-    int rc = pfast::dupload(&m->vp, vp, N, ctx->stream);
-    if (!rc) rc = pfast::dupload(&m->Z, Z, (size_t)NumofR * ncols * 3, ctx->stream);
+    int rc = pfast::dupload(ctx, &m->vp, vp, N);
+    if (!rc) rc = pfast::dupload(ctx, &m->Z, Z, (size_t)NumofR * ncols * 3);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) {
         pfast::set_error("upload of the 2D model failed");
         rc = PFAST_ERR_HIP;
@@ -153,8 +205,8 @@ int pfast_model2d_destroy(pfast_model2d *m)
     ARGCHK(m->nshots == 0, "model still has live shot sets: destroy them first");
     (void)hipStreamSynchronize(m->ctx->stream);
     m->ctx->nmodels--;
-    if (m->vp) (void)hipFree(m->vp);
-    if (m->Z) (void)hipFree(m->Z);
+    pfast::dfree(m->ctx, m->vp);
+    pfast::dfree(m->ctx, m->Z);
     delete m;
     return PFAST_OK;
 }
@@ -163,16 +215,28 @@ int pfast_shots2d_destroy(pfast_shots2d *s)
 {
     if (!s) return PFAST_OK;
     (void)hipStreamSynchronize(s->m->ctx->stream);
+    pfast::shots2d_free(s);
+    return PFAST_OK;
+}
+
+}  // extern "C"
+
+namespace pfast {
+
+void shots2d_free(pfast_shots2d *s)
+{
     s->m->nshots--;
     void *ptrs[] = {s->src, s->rec, s->srnum, s->dch0, s->nxy, s->TT_obs, s->TR_obs, s->TT,
                     s->TTd, s->TTu, s->LT, s->LR, s->Uw, s->tag, s->EEt, s->EEr, s->fcyc,
                     s->tcyc, s->rcyc, s->status, s->k2, s->gT, s->gR, s->gRT, s->WR,
                     s->fieldbuf};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    for (void *p : ptrs) pfast::dfree(s->m->ctx, p);
     delete s;
-    return PFAST_OK;
 }
+
+}  // namespace pfast
+
+extern "C" {
 
 int pfast_shots2d_create(pfast_model2d *m, int nsrc, const int *src, const int *srnum,
                          const int *rec, const double *nxy, const double *TT_obs,
@@ -208,27 +272,27 @@ int pfast_shots2d_create(pfast_model2d *m, int nsrc, const int *src, const int *
     const size_t F = (size_t)nsrc * N;
     int rc = 0;
     auto ok = [&](int r) { return (rc = r) == 0; };
-    if (!(ok(pfast::dupload(&s->src, src, 2 * (size_t)nsrc, st)) &&
-          ok(pfast::dupload(&s->srnum, srnum, nsrc, st)) &&
-          ok(pfast::dupload(&s->dch0, s->ch0.data(), nsrc + 1, st)) &&
-          ok(pfast::dalloc(&s->TT, F)) && ok(pfast::dalloc(&s->TTd, F * (R ? R : 1))) &&
-          ok(pfast::dalloc(&s->TTu, F * (R ? R : 1))) && ok(pfast::dalloc(&s->fcyc, (size_t)nsrc * (1 + 2 * R))) &&
-          ok(pfast::dalloc(&s->status, nsrc)) && ok(pfast::dalloc(&s->k2, (size_t)nsrc * ncols)) &&
-          ok(pfast::dalloc(&s->fieldbuf, N)))) {
+    if (!(ok(pfast::dupload(m->ctx, &s->src, src, 2 * (size_t)nsrc)) &&
+          ok(pfast::dupload(m->ctx, &s->srnum, srnum, nsrc)) &&
+          ok(pfast::dupload(m->ctx, &s->dch0, s->ch0.data(), nsrc + 1)) &&
+          ok(pfast::dalloc(m->ctx, &s->TT, F)) && ok(pfast::dalloc(m->ctx, &s->TTd, F * (R ? R : 1))) &&
+          ok(pfast::dalloc(m->ctx, &s->TTu, F * (R ? R : 1))) && ok(pfast::dalloc(m->ctx, &s->fcyc, (size_t)nsrc * (1 + 2 * R))) &&
+          ok(pfast::dalloc(m->ctx, &s->status, nsrc)) && ok(pfast::dalloc(m->ctx, &s->k2, (size_t)nsrc * ncols)) &&
+          ok(pfast::dalloc(m->ctx, &s->fieldbuf, N)))) {
         pfast_shots2d_destroy(s);
         return rc;
     }
     if (adj) {
-        if (!(ok(pfast::dupload(&s->rec, rec, 2 * (size_t)s->nrec, st)) &&
-              ok(pfast::dupload(&s->nxy, nxy, 2 * (size_t)s->nrec, st)) &&
-              ok(pfast::dupload(&s->TT_obs, TT_obs, s->nrec, st)) &&
-              ok(pfast::dupload(&s->TR_obs, TR_obs, (size_t)s->nrec * R, st)) &&
-              ok(pfast::dalloc(&s->LT, F)) && ok(pfast::dalloc(&s->LR, F)) &&
-              ok(pfast::dalloc(&s->Uw, F)) && ok(pfast::dalloc(&s->tag, F)) &&
-              ok(pfast::dalloc(&s->EEt, nsrc)) && ok(pfast::dalloc(&s->EEr, nsrc)) &&
-              ok(pfast::dalloc(&s->tcyc, nsrc)) && ok(pfast::dalloc(&s->rcyc, (size_t)nsrc * (R ? R : 1))) &&
-              ok(pfast::dalloc(&s->gT, N)) && ok(pfast::dalloc(&s->gR, N)) &&
-              ok(pfast::dalloc(&s->gRT, N)) && ok(pfast::dalloc(&s->WR, N)))) {
+        if (!(ok(pfast::dupload(m->ctx, &s->rec, rec, 2 * (size_t)s->nrec)) &&
+              ok(pfast::dupload(m->ctx, &s->nxy, nxy, 2 * (size_t)s->nrec)) &&
+              ok(pfast::dupload(m->ctx, &s->TT_obs, TT_obs, s->nrec)) &&
+              ok(pfast::dupload(m->ctx, &s->TR_obs, TR_obs, (size_t)s->nrec * R)) &&
+              ok(pfast::dalloc(m->ctx, &s->LT, F)) && ok(pfast::dalloc(m->ctx, &s->LR, F)) &&
+              ok(pfast::dalloc(m->ctx, &s->Uw, F)) && ok(pfast::dalloc(m->ctx, &s->tag, F)) &&
+              ok(pfast::dalloc(m->ctx, &s->EEt, nsrc)) && ok(pfast::dalloc(m->ctx, &s->EEr, nsrc)) &&
+              ok(pfast::dalloc(m->ctx, &s->tcyc, nsrc)) && ok(pfast::dalloc(m->ctx, &s->rcyc, (size_t)nsrc * (R ? R : 1))) &&
+              ok(pfast::dalloc(m->ctx, &s->gT, N)) && ok(pfast::dalloc(m->ctx, &s->gR, N)) &&
+              ok(pfast::dalloc(m->ctx, &s->gRT, N)) && ok(pfast::dalloc(m->ctx, &s->WR, N)))) {
             pfast_shots2d_destroy(s);
             return rc;
         }
@@ -248,46 +312,15 @@ int pfast_shots2d_run(pfast_shots2d *s, int mode, int MAXSwp)
 {
     ARGCHK(s && (mode == PFAST_FWD_ONLY || mode == PFAST_FWD_ADJ) && MAXSwp > 0, "bad argument");
     ARGCHK(mode == PFAST_FWD_ONLY || s->LT, "shot set created without receiver data");
-    pfast_model2d *m = s->m;
-    pfast_ctx *ctx = m->ctx;
-    hipStream_t st = ctx->stream;
-    const Grid2 g = grid_of(m);
-    const long N = (long)m->nrows * m->ncols;
-    const int R = m->NumofR;
-    s->have_fwd = s->have_adj = false;
-    pfast::time_begin(ctx, 2);
-    HIPCHK(pfast::launch_rt_fsm2d_f32(st, m->vp, s->TT, s->TTd, s->TTu, m->Z, s->src, s->nsrc, R,
-                                      MAXSwp, g, s->fcyc, s->status, s->k2));
-    pfast::time_end(ctx);
+    pfast_ctx *ctx = s->m->ctx;
     std::vector<int> status(s->nsrc);
-    HIPCHK(hipMemcpyAsync(status.data(), s->status, s->nsrc * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    int rc = pfast::issue_fwd2d(s, MAXSwp, status.data());
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     pfast::time_collect(ctx);
-    for (int i = 0; i < s->nsrc; i++)
-        if (status[i]) {
-            pfast::set_error(std::string("source ") + std::to_string(i) +
-                             (status[i] == 1 ? ": no k2 above the reflector in a column (the "
-                                               "reference exits, src/RT_pfsm2d.h:232,257,281)"
-                                             : ": negative reflected traveltime seed (the "
-                                               "reference exits, src/RT_pfsm2d.h:227,253,274)"));
-            return PFAST_ERR_FATAL;
-        }
-    s->have_fwd = true;
-    if (mode == PFAST_FWD_ONLY) return PFAST_OK;
-    pfast::time_begin(ctx, 2);
-    HIPCHK(pfast::launch_tadj2d_f32(st, s->LT, s->tag, s->TT, s->TT_obs, 1.0, s->rec, s->nxy,
-                                    s->dch0, s->srnum, s->src, s->nsrc, MAXSwp, g, s->EEt, s->tcyc));
-    pfast::time_end(ctx);
-    if (R > 0) {
-        pfast::time_begin(ctx, 2);
-        HIPCHK(pfast::launch_radj2d_f32(st, s->LR, s->Uw, s->tag, s->TTu, s->TR_obs, 1.0, s->rec,
-                                        s->nxy, s->dch0, s->srnum, s->src, s->nsrc, R, MAXSwp, g,
-                                        s->EEr, s->rcyc));
-        pfast::time_end(ctx);
-    } else {
-        HIPCHK(hipMemsetAsync(s->EEr, 0, s->nsrc * sizeof(double), st));
-    }
-    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = pfast::check_fwd2d(s, status.data())) || mode == PFAST_FWD_ONLY) return rc;
+    if ((rc = pfast::issue_adj2d(s, MAXSwp))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     pfast::time_collect(ctx);
     s->have_adj = true;
     return PFAST_OK;
@@ -301,10 +334,8 @@ int pfast_shots2d_gradient(pfast_shots2d *s, const double *WR, double *gT, doubl
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols;
     if (WR) HIPCHK(hipMemcpyAsync(s->WR, WR, N * sizeof(double), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(grad2d_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, s->LT,
-                       s->LR, m->vp, WR ? s->WR : (const double *)nullptr, s->gT, s->gR, s->gRT,
-                       N, s->nsrc, m->NumofR > 0 ? 1 : 0);
-    HIPCHK(hipGetLastError());
+    const int rc = pfast::issue_grad2d(s, WR ? s->WR : nullptr);
+    if (rc) return rc;
     if (gT) HIPCHK(hipMemcpyAsync(gT, s->gT, N * sizeof(double), hipMemcpyDeviceToHost, st));
     if (gR) HIPCHK(hipMemcpyAsync(gR, s->gR, N * sizeof(double), hipMemcpyDeviceToHost, st));
     if (gRT) HIPCHK(hipMemcpyAsync(gRT, s->gRT, N * sizeof(double), hipMemcpyDeviceToHost, st));

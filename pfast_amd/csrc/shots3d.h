@@ -1,0 +1,84 @@
+// shots3d.h -- the batched 3D model and shot-set objects and the pieces of one
+// forward+adjoint solve, shared by the C-ABI entry points and the request server.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "batch3d.h"
+#include "context.h"
+
+struct pfast_model3d {
+    pfast_ctx *ctx;
+    int nrows, ncols, nz;
+    double dx, dy, dz;
+    bool uniform;
+    double *vp = nullptr;  // N doubles
+    float *f = nullptr;    // N cell times / slowness
+    int nshots = 0;        // live shot sets bound to this model
+};
+
+struct pfast_shots3d {
+    pfast_model3d *m;
+    int nsrc, nchunk;
+    long nrec;
+    std::vector<int> src;    // 3*nsrc
+    std::vector<int> srnum;  // nsrc
+    std::vector<long> ch0;   // nsrc+1 prefix
+    int *srcnode = nullptr;  // [nchunk*C]
+    float *T = nullptr, *L = nullptr;
+    ChunkRef *chunks = nullptr;          // [nchunk] device table the sweep kernels read
+    std::vector<ChunkRef> hchunks;       // host copy
+    // receiver table (reference order) for picks and the single-source path
+    long *rnode = nullptr;
+    int *rsrc = nullptr;
+    int *rec3 = nullptr;
+    double *nxyz = nullptr, *Tobs = nullptr;
+    // picks grouped by source (misfit, fixed-node marks)
+    long *pnode = nullptr;
+    int *psrc = nullptr;
+    double *pobs = nullptr;
+    long *poff = nullptr;
+    int npick = 0;
+    // boundary-condition entries (last picked receiver per node, not at the source)
+    long *bnode = nullptr;
+    int *bsrc = nullptr;
+    double *bobs = nullptr, *bnrm = nullptr;
+    int nb = 0;
+    double *EE = nullptr, *grad = nullptr, *picks = nullptr, *fieldbuf = nullptr;
+    unsigned *changed = nullptr;
+    int *active = nullptr;
+    unsigned *hchanged = nullptr;
+    std::vector<int> fcyc, acyc;
+    bool have_adj_data = false;
+};
+
+namespace pfast {
+
+struct Grid3 {
+    int nrows, ncols, nz;
+    double dx, dy, dz;
+    bool uniform;
+};
+
+inline Grid3 grid_of(const pfast_model3d *m)
+{
+    return Grid3{m->nrows, m->ncols, m->nz, m->dx, m->dy, m->dz, m->uniform};
+}
+
+// One sweep cycle (the 8 orderings, each as one launch per tile anti-diagonal) on
+// ctx's stream over the chunk ids fwd[0..nfwd) (forward sweeps) and adj[0..nadj)
+// (adjoint sweeps) of the chunk table `chunks`, all of grid g; device arrays.
+// changed[id] collects, per chunk id, the sources that changed in this cycle.
+int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                  int nfwd, const int *adj, int nadj, unsigned *changed);
+
+// Stages of pfast_shots3d_run on the model's stream.
+int fwd_begin3d(pfast_shots3d *s);  // traveltime fields: 0 at the source, INF elsewhere
+int adj_begin3d(pfast_shots3d *s);  // misfit, lambda = 0, receiver conditions, marks
+int adj_end3d(pfast_shots3d *s);    // unmark, gradient reduction over sources
+
+// Free a shot set without synchronising: no work on the device may still use it.
+void shots3d_free(pfast_shots3d *s);
+
+}  // namespace pfast
