@@ -108,6 +108,32 @@ def measured_traffic(kernel, sweep_name, args):
     return t.get("bytes_per_launch", {}).get(kernel)
 
 
+def serving(args):
+    """p50/p99 latency and throughput of the request server under a Poisson stream
+    of table and gradient requests on the five serving grids."""
+    import pfast_amd
+    from pfast_amd import loadgen as LG
+    srv = pfast_amd.Server(0, int(args.serve_budget_gib * (1 << 30)))
+    t0 = time.perf_counter()
+    models = LG.load_models(srv, per_grid=2)
+    log(f"serving: 10 resident models in {time.perf_counter() - t0:.1f} s; warm-up stream ...")
+    LG.poisson_run(srv, models, 4.0, 2.0, seed=1)
+    log(f"serving: Poisson stream, {args.serve_rate} requests/s for {args.serve_seconds} s ...")
+    recs = LG.poisson_run(srv, models, args.serve_rate, args.serve_seconds, seed=7, log=log)
+    info = srv.info()
+    srv.close()
+    res = LG.summarize(recs)
+    res.update({"rate_hz": args.serve_rate, "arrival_window_s": args.serve_seconds,
+                "workload": "Poisson arrivals; grid uniform over 2D 101x51, 2D 401x101, 3D 65^3, "
+                            "3D 129x129x65, 3D 321x321x161 (h=25 m, 2 resident seeded models "
+                            "each); 1-64 surface sources; table or gradient (32 receivers per "
+                            "source); latency = submission to result resident on the device",
+                "sweep_cycles": info["cycles"], "node_sweeps": info["node_sweeps"]})
+    log(f"serving: {res['requests']} requests, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
+        f"{res['requests_per_s']} requests/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,12 +152,19 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (3 times), the "
                          "known-size kernel that scales the HBM byte counters of a profile")
+    ap.add_argument("--serve-rate", type=float, default=8.0,
+                    help="Poisson arrival rate (requests/s) of the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="arrival window of the serving measurement")
+    ap.add_argument("--serve-budget-gib", type=float, default=200.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the batched path keeps every source of a shot set "
-              "on a single MI355X and has no multi-GPU mode (see DESIGN.md for how it would "
-              "shard).  Run with --gpus 1.", file=sys.stderr)
+              "on a single MI355X and has no multi-GPU mode (across GPUs the sources would be "
+              "split, with one gradient all-reduce per evaluation).  Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import pfast_amd
@@ -263,6 +296,8 @@ def main():
     shots.close()
     model.close()
     ctx.close()
+    if not args.no_serve:
+        out["serving"] = serving(args)
     print(json.dumps(out), flush=True)
 
 

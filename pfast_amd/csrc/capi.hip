@@ -591,8 +591,14 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
 
 namespace pfast {
 
-int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed)
+int diagonals3d(const pfast_ctx *ctx, const Grid3 &g)
+{
+    const SweepVariant &sv = sweep_variant(ctx->sweep);
+    return (g.nrows + sv.ti - 1) / sv.ti + (g.ncols + sv.tj - 1) / sv.tj - 1;
+}
+
+int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1)
 {
     hipStream_t st = ctx->stream;
     const SweepVariant &sv = sweep_variant(ctx->sweep);
@@ -618,21 +624,30 @@ int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const 
     aa.active = adj;
     aa.nact = nadj;
     aa.nact_p8 = (nadj + 7) / 8 * 8;
-    for (int o = 0; o < 8; o++) {
-        const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
-        for (int D = 0; D <= nti + ntj - 2; D++) {
-            const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
-            if (nfwd) {
-                time_begin(ctx, 0);
-                HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
-                time_end(ctx);
-            }
-            if (nadj) {
-                time_begin(ctx, 1);
-                HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
-                time_end(ctx);
-            }
+    const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+    for (int D = D0; D < D1; D++) {
+        const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
+        if (nfwd) {
+            time_begin(ctx, 0);
+            HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
+            time_end(ctx);
         }
+        if (nadj) {
+            time_begin(ctx, 1);
+            HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
+            time_end(ctx);
+        }
+    }
+    return PFAST_OK;
+}
+
+int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0, int o1)
+{
+    const int nd = diagonals3d(ctx, g);
+    for (int o = o0; o < o1; o++) {
+        int rc = issue_diagonals3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, nd);
+        if (rc) return rc;
     }
     return PFAST_OK;
 }

@@ -10,6 +10,10 @@
 //     leave at cycle boundaries (continuous batching): every chunk sees exactly the
 //     sequence of full cycles pfast_shots3d_run gives it alone, so its traveltimes,
 //     lambda, gradient and misfit are bit-identical to a standalone run.
+//     A cycle is issued in slices of kSlice tile anti-diagonals with at most two
+//     slices queued, so that the streams of other groups and of 2D requests, which
+//     may share a hardware queue with a large group, wait for milliseconds of its
+//     work rather than for a whole cycle.
 //   * 2D requests run one launch per stage (forward, adjoints) on one of the 2D
 //     streams; different models use different streams and overlap.
 //   * Admission: first fit in arrival order while the device memory of the
@@ -39,6 +43,8 @@
 namespace {
 
 constexpr int kStreams2D = 2;
+constexpr int kSlice = 8;     // tile anti-diagonals per queued slice of a group cycle
+constexpr int kQueued = 2;    // slices of a group cycle queued at once
 constexpr int kMaxSrc2D = 1 << 16;  // per-request source limit of the 2D status staging
 
 enum Phase { PH_NONE = 0, PH_FWD, PH_ADJ, PH_FINISH };
@@ -86,8 +92,10 @@ struct Group {
     int *dact = nullptr, *hact = nullptr;
     unsigned *dchanged = nullptr, *hchanged = nullptr;
     bool dirty = true;
-    hipEvent_t ev = nullptr;
-    bool inflight = false;
+    hipEvent_t ev[kQueued] = {};  // end of the queued slices, by slice number mod kQueued
+    bool inflight = false;        // a cycle is in progress
+    int o = 0, D = 0;             // next slice: ordering, first anti-diagonal
+    int issued = 0, done = 0, nfwd = 0, nadj = 0;
     std::vector<int> cyc_ids;  // chunk ids of the cycle in flight
     std::set<long> cyc_reqs;   // requests of the cycle in flight
 };
@@ -196,11 +204,14 @@ int group_of(pfast_server *S, const pfast::Grid3 &g, int *out)
     int rc = pfast_ctx_create(S->device, &G->ctx);
     if (rc) return rc;
     G->ctx->pool = &S->pool;
-    if (hipEventCreateWithFlags(&G->ev, hipEventDisableTiming) != hipSuccess) {
-        pfast_ctx_destroy(G->ctx);
-        pfast::set_error("hipEventCreate failed");
-        return PFAST_ERR_HIP;
-    }
+    for (auto &e : G->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            for (auto &f : G->ev)
+                if (f) (void)hipEventDestroy(f);
+            pfast_ctx_destroy(G->ctx);
+            pfast::set_error("hipEventCreate failed");
+            return PFAST_ERR_HIP;
+        }
     S->groups.push_back(std::move(G));
     *out = (int)S->groups.size() - 1;
     return PFAST_OK;
@@ -309,7 +320,32 @@ void try_admit(pfast_server *S)
     }
 }
 
-// One sweep cycle over the live chunks of group G.
+// Keep at most kQueued slices of the cycle in flight queued on the group's stream;
+// the last one is followed by the copy of the changed flags.
+int pump_cycle(Group &G)
+{
+    hipStream_t st = G.ctx->stream;
+    const int nd = pfast::diagonals3d(G.ctx, G.g);
+    while (G.o < 8 && G.issued - G.done < kQueued) {
+        const int D1 = std::min(G.D + kSlice, nd);
+        int rc = pfast::issue_diagonals3d(G.ctx, G.g, G.dtable, G.dact, G.nfwd, G.dact + G.nfwd,
+                                          G.nadj, G.dchanged, G.o, G.D, D1);
+        if (rc) return rc;
+        G.D = D1;
+        if (G.D == nd) {
+            G.o++;
+            G.D = 0;
+        }
+        if (G.o == 8)
+            HIPCHK(hipMemcpyAsync(G.hchanged, G.dchanged, G.table.size() * sizeof(unsigned),
+                                  hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(G.ev[G.issued % kQueued], st));
+        G.issued++;
+    }
+    return PFAST_OK;
+}
+
+// Start a sweep cycle over the live chunks of group G.
 int issue_cycle(pfast_server *S, Group &G)
 {
     const int n = (int)G.table.size();
@@ -340,17 +376,15 @@ int issue_cycle(pfast_server *S, Group &G)
     const int na = (int)(fwd.size() + adj.size());
     HIPCHK(hipMemcpyAsync(G.dact, G.hact, na * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(G.dchanged, 0, n * sizeof(unsigned), st));
-    rc = pfast::issue_cycle3d(G.ctx, G.g, G.dtable, G.dact, (int)fwd.size(), G.dact + fwd.size(),
-                              (int)adj.size(), G.dchanged);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(G.hchanged, G.dchanged, n * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(G.ev, st));
+    G.nfwd = (int)fwd.size();
+    G.nadj = (int)adj.size();
+    G.o = G.D = G.issued = G.done = 0;
     G.cyc_ids = fwd;
     G.cyc_ids.insert(G.cyc_ids.end(), adj.begin(), adj.end());
     G.inflight = true;
     S->cycles++;
     S->node_sweeps += 8.0 * G.g.nrows * G.g.ncols * G.g.nz * (double)real;
-    return PFAST_OK;
+    return pump_cycle(G);
 }
 
 // A finished cycle: convergence per chunk, phase ends per request.
@@ -482,7 +516,14 @@ int progress_once(pfast_server *S, int *ndone)
 {
     for (auto &G : S->groups) {
         if (!G->inflight) continue;
-        const hipError_t q = hipEventQuery(G->ev);
+        hipError_t q = hipSuccess;
+        while (G->done < G->issued && (q = hipEventQuery(G->ev[G->done % kQueued])) == hipSuccess)
+            G->done++;
+        if (q == hipSuccess && (G->o < 8 || G->done < G->issued)) {
+            int rc = pump_cycle(*G);
+            if (rc) return rc;
+            continue;
+        }
         if (q == hipErrorNotReady) continue;
         G->inflight = false;
         if (q != hipSuccess) {
@@ -622,7 +663,8 @@ int pfast_server_destroy(pfast_server *S)
             (void)hipHostFree(G->hact);
             (void)hipHostFree(G->hchanged);
         }
-        if (G->ev) (void)hipEventDestroy(G->ev);
+        for (auto &e : G->ev)
+            if (e) (void)hipEventDestroy(e);
         pfast_ctx_destroy(G->ctx);
     }
     S->groups.clear();

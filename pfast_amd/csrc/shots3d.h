@@ -66,12 +66,17 @@ inline Grid3 grid_of(const pfast_model3d *m)
     return Grid3{m->nrows, m->ncols, m->nz, m->dx, m->dy, m->dz, m->uniform};
 }
 
-// One sweep cycle (the 8 orderings, each as one launch per tile anti-diagonal) on
-// ctx's stream over the chunk ids fwd[0..nfwd) (forward sweeps) and adj[0..nadj)
-// (adjoint sweeps) of the chunk table `chunks`, all of grid g; device arrays.
-// changed[id] collects, per chunk id, the sources that changed in this cycle.
+// Orderings [o0, o1) of one sweep cycle (8 orderings, each as one launch per tile
+// anti-diagonal) on ctx's stream over the chunk ids fwd[0..nfwd) (forward sweeps)
+// and adj[0..nadj) (adjoint sweeps) of the chunk table `chunks`, all of grid g;
+// device arrays.  changed[id] collects, per chunk id, the sources that changed.
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed);
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0 = 0, int o1 = 8);
+// Tile anti-diagonals [D0, D1) of ordering o (same arguments otherwise).
+int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1);
+// Tile anti-diagonals of one ordering under ctx's sweep variant.
+int diagonals3d(const pfast_ctx *ctx, const Grid3 &g);
 
 // Stages of pfast_shots3d_run on the model's stream.
 int fwd_begin3d(pfast_shots3d *s);  // traveltime fields: 0 at the source, INF elsewhere

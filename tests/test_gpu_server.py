@@ -217,3 +217,24 @@ def test_admission_by_footprint():
     srv.remove_model(ids[0])
     assert srv.info()["in_use"] == 0
     srv.close()
+
+
+def test_poisson_stream_completes():
+    """A short Poisson stream over the small serving grids: every request finishes,
+    latencies are ordered and positive, and the server ends empty."""
+    import pfast_amd
+    from pfast_amd import loadgen as LG
+    grids = LG.SERVING_GRIDS[:3]
+    srv = pfast_amd.Server(0, 16 << 30)
+    models = LG.load_models(srv, grids=grids, per_grid=2)
+    resident = srv.info()["in_use"]
+    recs = LG.poisson_run(srv, models, 40.0, 2.0, seed=3)
+    res = LG.summarize(recs, grids)
+    print(res, flush=True)
+    assert res["requests"] == len(recs) > 20
+    assert 0 < res["p50_ms"] <= res["p99_ms"]
+    assert all(r["queue"] >= 0 and r["latency"] >= r["queue"] for r in recs)
+    srv.progress(0)
+    info = srv.info()
+    assert info["running"] == 0 and info["queued"] == 0 and info["in_use"] == resident
+    srv.close()
