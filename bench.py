@@ -152,6 +152,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (3 times), the "
                          "known-size kernel that scales the HBM byte counters of a profile")
+    ap.add_argument("--no-tile-skip", action="store_true",
+                    help="sweep every tile (exact tile skipping off; same results)")
+    ap.add_argument("--streams", type=int, default=1,
+                    help="streams a batched cycle splits its source chunks over (1..4)")
     ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="Poisson arrival rate (requests/s) of the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
@@ -190,6 +194,8 @@ def main():
     log(f"workload generated in {time.perf_counter() - t0:.1f}s: grid {vp.shape}, "
         f"{args.nsrc} sources x {args.nrec} receivers")
     ctx = pfast_amd.Context(0)
+    ctx.set_streams(args.streams)
+    ctx.set_tile_skip(not args.no_tile_skip)
     ctx.set_sweep(args.sweep)
     sweep_name = pfast_amd.sweep_variants()[args.sweep]
     log(f"sweep kernels: {sweep_name}")

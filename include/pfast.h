@@ -159,6 +159,14 @@ int pfast_ctx_timing_reset(pfast_ctx *ctx);
  * past the last one. */
 int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant);
 const char *pfast_sweep_name(int variant);
+/* Streams (1..4) over which a batched 3D cycle splits its active source chunks, so
+ * that the launches of one stream fill the idle tail of another's; results are
+ * the same for every count (each chunk's sweeps run in the same order). */
+int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams);
+/* Exact tile skipping of the batched 3D sweeps (on by default): a tile whose last
+ * sweep in an ordering changed nothing, and whose nodes and neighbour tiles have
+ * not changed since, is not swept again in that ordering; results are the same. */
+int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable);
 
 /* ---------------------------------------------------------------------------------
  * Batched 2D shot sets: every source of a 2D survey resident on one GPU (FP32

@@ -165,6 +165,14 @@ class Context:
         """Select the batched sweep kernel variant (0 = default); see sweep_variants()."""
         check(lib().pfast_ctx_set_sweep(self.h, int(variant)))
 
+    def set_tile_skip(self, enable=True):
+        """Exact tile skipping of the batched 3D sweeps (default on; same results)."""
+        check(lib().pfast_ctx_set_tile_skip(self.h, 1 if enable else 0))
+
+    def set_streams(self, n):
+        """Split batched 3D cycles over n streams (1..4); results do not change."""
+        check(lib().pfast_ctx_set_streams(self.h, int(n)))
+
     def timing_read(self, kind):
         """(device ms, launches, node-sweeps) of sweep kind 0 (forward) / 1 (adjoint)."""
         ms, n, w = C.c_double(0), C.c_long(0), C.c_double(0)

@@ -22,9 +22,17 @@ struct SlabArgs {
     int nact, nact_p8;    // active chunks; grid stride (multiple of 8 = XCD count)
     int nrows, ncols, nz, nti, ntj;
     float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
+    // Exact tile skipping (column-window kernels; NULL: off).  Per (chunk, tile):
+    // tmod = epoch of the last sweep that changed a node of the tile, tclean[o] =
+    // epoch of the last sweep in ordering o that changed nothing.  A tile whose
+    // clean sweep in this ordering is newer than every change of it and of its four
+    // neighbour tiles would see the same inputs again and change nothing: skipped.
+    int *tmod, *tclean;
+    int epoch, o;         // this launch's epoch (increasing within a phase), ordering
 };
 
-// A sweep kernel variant: schedule (0 column window, 1 slab ring, 2 loader wave),
+// A sweep kernel variant: schedule (0 column window, 1 slab ring, 2 loader wave,
+// 3 vector lanes: a chunk's sources in one lane),
 // tile rows x columns, batch (column window), prefetch (slab ring) or forward
 // lookahead (loader wave) depth.
 struct SweepVariant {

@@ -507,6 +507,46 @@ __device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SlabArgs &a, c
     return l;
 }
 
+// Tile index of (chunk, tile) in the skip records, or -1 without skipping.
+template <int TI, int TJ>
+__device__ __forceinline__ int skip_index(const SlabArgs &a, const TileCtx &t)
+{
+    return a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
+}
+
+// The sweep of tile ix in this ordering would change nothing (see SlabArgs).
+template <int TI, int TJ>
+__device__ __forceinline__ bool skip_tile(const SlabArgs &a, const TileCtx &t, int ix)
+{
+    if (ix < 0) return false;
+    const int cl = a.tclean[ix * 8 + a.o];
+    if (cl <= 0) return false;
+    const int ti = t.i0 / TI, tj = t.j0 / TJ;
+    int m = a.tmod[ix];
+    if (ti > 0) m = max(m, a.tmod[ix - a.ntj]);
+    if (ti < a.nti - 1) m = max(m, a.tmod[ix + a.ntj]);
+    if (tj > 0) m = max(m, a.tmod[ix - 1]);
+    if (tj < a.ntj - 1) m = max(m, a.tmod[ix + 1]);
+    return m < cl;
+}
+
+// Record the outcome of this tile sweep: any wave of the block changed a node.
+__device__ __forceinline__ void record_tile(const SlabArgs &a, int ix, unsigned long long wave_vote,
+                                            int *any)
+{
+    if (ix < 0) return;
+    if ((threadIdx.x & 63) == 0 && wave_vote) atomicOr(any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (*any) {
+            a.tmod[ix] = a.epoch;
+            a.tclean[ix * 8 + a.o] = 0;
+        } else {
+            a.tclean[ix * 8 + a.o] = a.epoch;
+        }
+    }
+}
+
 template <int TI, int TJ, int C, int L, bool UNI>
 __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1, int s2, int s3)
 {
@@ -516,8 +556,12 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     constexpr int PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
 
+    __shared__ int any_change;
+
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live) return;
+    const int six = skip_index<TI, TJ>(a, t);
+    if (skip_tile<TI, TJ>(a, t, six)) return;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
@@ -526,7 +570,10 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
     const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
     const float INF = PFAST_INF_F;
-    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF;
+    if (tid == 0) {
+        pub[2 * NT] = pub[PB + 2 * NT] = INF;
+        any_change = 0;
+    }
     // steps p at which this thread updates its node: pipj <= p < pipj + nz, except
     // the level of its own source
     const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
@@ -623,6 +670,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     }
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+    record_tile(a, six, bm, &any_change);
 }
 
 // Adjoint, column window: lambda and traveltime travel together as float2 (the
@@ -637,8 +685,12 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][thread], INF slot
     __shared__ float2 pub[2 * PB];
 
+    __shared__ int any_change;
+
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live) return;
+    const int six = skip_index<TI, TJ>(a, t);
+    if (skip_tile<TI, TJ>(a, t, six)) return;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
@@ -653,7 +705,10 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
     const unsigned st = l.step;
     const float wx = a.wx, wy = a.wy, wz = a.wz;
-    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
+    if (tid == 0) {
+        pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
+        any_change = 0;
+    }
 
     auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
     auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
@@ -697,38 +752,35 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
             P[tid] = vprev;
             P[NT + tid] = vnext;
             lds_barrier();
+            // branch-free: the (lambda, T) pairs of the neighbours come in one LDS read
+            // each, the update is computed in every lane and selected afterwards
             const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            float nl = vcur.x;
-            bool c = false;
-            if (p >= plo && p < phi && !signbit(vcur.y)) {
-                const float2 xm = l.uxm ? q[1][u] : pxm;
-                const float2 xp = l.uxp ? q[1][u] : pxp;
-                const float2 ym = l.uym ? q[2][u] : pym;
-                const float2 yp = l.uyp ? q[2][u] : pyp;
-                // depth k-1 / k+1 by absolute depth (levels m-1 / m+1 are depths
-                // k-s3 / k+s3): every ordering sums the terms in one order, so a
-                // stationary node stays bitwise stationary
-                const float2 zm = S3POS ? vprev : vnext;
-                const float2 zp = S3POS ? vnext : vprev;
-                const float tc = vcur.y;
-                const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
-                            typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
-                float nw = 0.0f;
-                const float I = PFAST_INF_F;
-                if (!(tc >= I || txm >= I || txp >= I || tym >= I || typ >= I || tzm >= I ||
-                      tzp >= I)) {
-                    float den = 0.0f;
-                    float num = adj_term(tc, txp, xp.x, wx, den);
-                    num += adj_term(tc, txm, xm.x, wx, den);
-                    num += adj_term(tc, typ, yp.x, wy, den);
-                    num += adj_term(tc, tym, ym.x, wy, den);
-                    num += adj_term(tc, tzp, zp.x, wz, den);
-                    num += adj_term(tc, tzm, zm.x, wz, den);
-                    nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
-                }
-                c = nw != vcur.x;
-                nl = nw;
-            }
+            const float2 xm = l.uxm ? q[1][u] : pxm;
+            const float2 xp = l.uxp ? q[1][u] : pxp;
+            const float2 ym = l.uym ? q[2][u] : pym;
+            const float2 yp = l.uyp ? q[2][u] : pyp;
+            // depth k-1 / k+1 by absolute depth (levels m-1 / m+1 are depths
+            // k-s3 / k+s3): every ordering sums the terms in one order, so a
+            // stationary node stays bitwise stationary
+            const float2 zm = S3POS ? vprev : vnext;
+            const float2 zp = S3POS ? vnext : vprev;
+            const float tc = vcur.y;
+            const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
+                        typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
+            const float I = PFAST_INF_F;
+            const bool anyinf = tc >= I || txm >= I || txp >= I || tym >= I || typ >= I ||
+                                tzm >= I || tzp >= I;
+            float den = 0.0f;
+            float num = adj_term(tc, txp, xp.x, wx, den);
+            num += adj_term(tc, txm, xm.x, wx, den);
+            num += adj_term(tc, typ, yp.x, wy, den);
+            num += adj_term(tc, tym, ym.x, wy, den);
+            num += adj_term(tc, tzp, zp.x, wz, den);
+            num += adj_term(tc, tzm, zm.x, wz, den);
+            const float nw = !anyinf && fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+            const bool act = p >= plo && p < phi && !signbit(vcur.y);
+            const bool c = act && nw != vcur.x;
+            const float nl = act ? nw : vcur.x;
             chg[u] = c;
             changed |= c;
             sval[u] = nl;
@@ -751,6 +803,315 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     }
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+    record_tile(a, six, bm, &any_change);
+}
+
+// ============================================================ vector-lane sweeps
+// Fourth schedule ("vector lanes"): the column-window sweep with one lane per
+// column and the chunk's C = 4 sources of a node (its 16-byte record) in that lane.
+// Loads, the LDS exchange and stores move whole records; the four updates of a node
+// run as independent instruction chains, and what depends only on the node
+// (addresses, masks, neighbour selection, the cell time) is done once for the
+// four.  An 8x8 tile is one wave.  Same steps and the same per-node arithmetic as
+// fwd3d_col / adj3d_col, so the same results bit for bit; a record is stored when
+// any of its sources changed (the others rewrite their own value).
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 bload4(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ void bstore4(rsrc_t r, unsigned off, f32x4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+}
+
+// Column lane of a vector tile: indices and LDS slots as a one-source column lane,
+// field offsets in 16-byte node records (cell-time offsets stay 4-byte).
+template <int TI, int TJ>
+__device__ __forceinline__ ColLane<TI, TJ, 1> make_vec_lane(const SlabArgs &a, const TileCtx &t,
+                                                            int s1, int s2, int s3)
+{
+    ColLane<TI, TJ, 1> l = make_col_lane<TI, TJ, 1>(a, t, s1, s2, s3);
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
+    const unsigned NONE = 0x80000000u;  // out of range at every level (never loaded)
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    const int hxj = l.uxm ? t.j0 - 1 : t.j0 + t.TJe;
+    const int hyi = l.uym ? t.i0 - 1 : t.i0 + t.TIe;
+    l.oI = 16u * (k0 * plane + l.node);
+    l.oHX = l.uxm || l.uxp ? 16u * (k0 * plane + (unsigned)gi * a.ncols + hxj) : NONE;
+    l.oHY = l.uym || l.uyp ? 16u * (k0 * plane + (unsigned)hyi * a.ncols + gj) : NONE;
+    l.step = s3 > 0 ? 16u * plane : 0u - 16u * plane;
+    return l;
+}
+
+// per-chunk changed bits of a wave: bit c iff some lane changed source c
+__device__ __forceinline__ void vote_changed(const SlabArgs &a, int chunk, unsigned changed)
+{
+    unsigned bits = 0;
+#pragma unroll
+    for (int c = 0; c < PFAST_CHUNK; c++)
+        if (__ballot((changed >> c) & 1u)) bits |= 1u << c;
+    if ((threadIdx.x & 63) == 0 && bits) atomicOr(&a.changed[chunk], bits);
+}
+
+template <int TI, int TJ, int L, bool UNI>
+__global__ __launch_bounds__(TI *TJ) void fwd3d_vec(SlabArgs a, int D, int s1, int s2, int s3)
+{
+    constexpr int NT = TI * TJ, C = PFAST_CHUNK;
+    static_assert(C == 4, "one 16-byte record per lane");
+    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
+    constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][lane], INF slot
+    __shared__ f32x4 pub[2 * PB];
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const ColLane<TI, TJ, 1> l = make_vec_lane<TI, TJ>(a, t, s1, s2, s3);
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], 16u * plane * (unsigned)nz);
+    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
+    const float INF = PFAST_INF_F;
+    const f32x4 INF4 = {INF, INF, INF, INF};
+    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF4;
+    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
+    int psrc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int sn = a.chunks[t.chunk].srcnode[c];
+        psrc[c] = -1;
+        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
+            const int sk = (int)((unsigned)sn / plane);
+            psrc[c] = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
+        }
+    }
+    const unsigned st = l.step, stF = l.stepF;
+    auto load_batch = [&](int pb, f32x4 *ow, f32x4 *hx, f32x4 *hy, float *fv) {
+        const int mb = pb - l.pipj;
+        const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const f32x4 o = bload4(RF, l.oI + dm + (unsigned)(u + 2) * st);
+            ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF4;
+            hx[u] = bload4(RF, l.oHX + dm + (unsigned)u * st);
+            hy[u] = bload4(RF, l.oHY + dm + (unsigned)u * st);
+            fv[u] = bload(RS, l.oF + dmf + (unsigned)u * stF);
+        }
+    };
+    const int m0 = -l.pipj;
+    f32x4 vprev = INF4;
+    f32x4 vcur = l.valid && m0 == 0 ? bload4(RF, l.oI) : INF4;
+    f32x4 vnext = l.valid && m0 >= -1 ? bload4(RF, l.oI + (unsigned)(m0 + 1) * st) : INF4;
+    f32x4 qa[3][L], qb[3][L];
+    float fa[L], fb[L];
+    load_batch(0, qa[0], qa[1], qa[2], fa);
+    __syncthreads();  // INF slot visible
+
+    unsigned changed = 0;
+    f32x4 sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = INF4;
+        chg[u] = false;
+    }
+    auto batch = [&](int pb, f32x4 (*q)[L], const float *qf, f32x4 (*nq)[L], float *nqf) {
+        wait_vm_all();
+        {  // records changed in the previous batch (steps pb-L .. pb-1)
+            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+            for (int u = 0; u < L; u++)
+                if (chg[u]) bstore4(RF, l.oI + d + (unsigned)u * st, sval[u]);
+        }
+        load_batch(pb + L, nq[0], nq[1], nq[2], nqf);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int p = pb + u;
+            f32x4 *P = pub + (u & 1) * PB;
+            P[tid] = vprev;
+            P[NT + tid] = vnext;
+            lds_barrier();
+            const f32x4 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
+            const f32x4 xm = l.uxm ? q[1][u] : pxm, xp = l.uxp ? q[1][u] : pxp;
+            const f32x4 ym = l.uym ? q[2][u] : pym, yp = l.uyp ? q[2][u] : pyp;
+            const bool live = p >= plo && p < phi;
+            f32x4 nv = vcur;
+            unsigned cm = 0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const float xa = fminf(xm[c], xp[c]), ya = fminf(ym[c], yp[c]);
+                const float za = fminf(vprev[c], vnext[c]);
+                float qv;
+                if (UNI) qv = eik3d_f32_uniform(xa, ya, za, qf[u]);
+                else qv = eik3d_f32_general(xa, ya, za, qf[u], a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
+                const bool cc = live && p != psrc[c] && qv < vcur[c];
+                nv[c] = cc ? qv : vcur[c];
+                cm |= cc ? 1u << c : 0u;
+            }
+            chg[u] = cm != 0;
+            changed |= cm;
+            sval[u] = nv;
+            vprev = nv;
+            vcur = vnext;
+            vnext = q[0][u];
+        }
+    };
+    const int pend = nz + t.span;
+    for (int pb = 0; pb < pend; pb += 2 * L) {
+        batch(pb, qa, fa, qb, fb);
+        batch(pb + L, qb, fb, qa, fa);
+    }
+    {
+        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
+        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore4(RF, l.oI + d + (unsigned)u * st, sval[u]);
+    }
+    vote_changed(a, t.chunk, changed);
+}
+
+template <int TI, int TJ, int L, bool S3POS>
+__global__ __launch_bounds__(TI *TJ) void adj3d_vec(SlabArgs a, int D, int s1, int s2)
+{
+    constexpr int NT = TI * TJ, C = PFAST_CHUNK;
+    constexpr int s3 = S3POS ? 1 : -1;
+    static_assert(C == 4, "one 16-byte record per lane");
+    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
+    constexpr int PB = 2 * NT + 1;
+    __shared__ f32x4 pubL[2 * PB];  // lambda records
+    __shared__ f32x4 pubT[2 * PB];  // traveltime records (negative: receiver node)
+
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const ColLane<TI, TJ, 1> l = make_vec_lane<TI, TJ>(a, t, s1, s2, s3);
+    const int tid = threadIdx.x;
+    const int nz = a.nz;
+    const unsigned plane = (unsigned)a.nrows * a.ncols;
+    const unsigned fbytes = 16u * plane * (unsigned)nz;
+    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
+    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
+    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
+    const bool col_interior =
+        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
+    const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
+    const unsigned st = l.step;
+    const float wx = a.wx, wy = a.wy, wz = a.wz;
+    const float I = PFAST_INF_F;
+    const f32x4 Z4 = {0.0f, 0.0f, 0.0f, 0.0f}, INF4 = {I, I, I, I};
+    if (tid == 0) {
+        pubL[2 * NT] = pubL[PB + 2 * NT] = Z4;
+        pubT[2 * NT] = pubT[PB + 2 * NT] = INF4;
+    }
+    auto load_batch = [&](int pb, f32x4 (*q)[L]) {
+        const unsigned dm = (unsigned)(pb - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const unsigned o = l.oI + dm + (unsigned)(u + 2) * st;
+            const unsigned ox = l.oHX + dm + (unsigned)u * st, oy = l.oHY + dm + (unsigned)u * st;
+            q[0][u] = bload4(RL, o);
+            q[1][u] = bload4(RT, o);
+            q[2][u] = bload4(RL, ox);
+            q[3][u] = bload4(RT, ox);
+            q[4][u] = bload4(RL, oy);
+            q[5][u] = bload4(RT, oy);
+        }
+    };
+    const int m0 = -l.pipj;
+    f32x4 pL = Z4, pT = Z4;  // level m-1
+    f32x4 cL = l.valid && m0 == 0 ? bload4(RL, l.oI) : Z4;
+    f32x4 cT = l.valid && m0 == 0 ? bload4(RT, l.oI) : Z4;
+    const unsigned on = l.oI + (unsigned)(m0 + 1) * st;
+    f32x4 nL = l.valid && m0 >= -1 ? bload4(RL, on) : Z4;
+    f32x4 nT = l.valid && m0 >= -1 ? bload4(RT, on) : Z4;
+    f32x4 qa[6][L], qb[6][L];
+    load_batch(0, qa);
+    __syncthreads();
+
+    unsigned changed = 0;
+    f32x4 sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = Z4;
+        chg[u] = false;
+    }
+    auto batch = [&](int pb, f32x4 (*q)[L], f32x4 (*nq)[L]) {
+        wait_vm_all();
+        {
+            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+            for (int u = 0; u < L; u++)
+                if (chg[u]) bstore4(RL, l.oI + d + (unsigned)u * st, sval[u]);
+        }
+        load_batch(pb + L, nq);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int p = pb + u;
+            f32x4 *PL = pubL + (u & 1) * PB, *PT = pubT + (u & 1) * PB;
+            PL[tid] = pL;
+            PT[tid] = pT;
+            PL[NT + tid] = nL;
+            PT[NT + tid] = nT;
+            lds_barrier();
+            const f32x4 xmL = l.uxm ? q[2][u] : PL[l.rxm], xmT = l.uxm ? q[3][u] : PT[l.rxm];
+            const f32x4 xpL = l.uxp ? q[2][u] : PL[l.rxp], xpT = l.uxp ? q[3][u] : PT[l.rxp];
+            const f32x4 ymL = l.uym ? q[4][u] : PL[l.rym], ymT = l.uym ? q[5][u] : PT[l.rym];
+            const f32x4 ypL = l.uyp ? q[4][u] : PL[l.ryp], ypT = l.uyp ? q[5][u] : PT[l.ryp];
+            // depth k-1 / k+1 by absolute depth, as in adj3d_col
+            const f32x4 zmL = S3POS ? pL : nL, zmT = S3POS ? pT : nT;
+            const f32x4 zpL = S3POS ? nL : pL, zpT = S3POS ? nT : pT;
+            const bool live = p >= plo && p < phi;
+            f32x4 nl = cL;
+            unsigned cm = 0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const float tc = cT[c];
+                const float txm = fabsf(xmT[c]), txp = fabsf(xpT[c]), tym = fabsf(ymT[c]),
+                            typ = fabsf(ypT[c]), tzm = fabsf(zmT[c]), tzp = fabsf(zpT[c]);
+                const bool anyinf = tc >= I || txm >= I || txp >= I || tym >= I || typ >= I ||
+                                    tzm >= I || tzp >= I;
+                float den = 0.0f;
+                float num = adj_term(tc, txp, xpL[c], wx, den);
+                num += adj_term(tc, txm, xmL[c], wx, den);
+                num += adj_term(tc, typ, ypL[c], wy, den);
+                num += adj_term(tc, tym, ymL[c], wy, den);
+                num += adj_term(tc, tzp, zpL[c], wz, den);
+                num += adj_term(tc, tzm, zmL[c], wz, den);
+                const float nw = !anyinf && fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+                const bool act = live && !signbit(tc);
+                const bool cc = act && nw != cL[c];
+                nl[c] = act ? nw : cL[c];
+                cm |= cc ? 1u << c : 0u;
+            }
+            chg[u] = cm != 0;
+            changed |= cm;
+            sval[u] = nl;
+            pL = nl;
+            pT = cT;
+            cL = nL;
+            cT = nT;
+            nL = q[0][u];
+            nT = q[1][u];
+        }
+    };
+    const int pend = nz + t.span;
+    for (int pb = 0; pb < pend; pb += 2 * L) {
+        batch(pb, qa, qb);
+        batch(pb + L, qb, qa);
+    }
+    {
+        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
+        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore4(RL, l.oI + d + (unsigned)u * st, sval[u]);
+    }
+    vote_changed(a, t.chunk, changed);
 }
 
 // ============================================================ loader-wave sweeps
@@ -1236,6 +1597,17 @@ static const SweepVariant kVariants[] = {
     {0, 8, 8, 2, "column-window 8x8 L2"},
     {0, 8, 4, 4, "column-window 8x4 L4"},
     {2, 8, 8, 12, "loader-wave 8x8 A12/8"},
+    {0, 4, 4, 6, "column-window 4x4 L6"},
+    {0, 4, 4, 4, "column-window 4x4 L4"},
+    {0, 4, 8, 6, "column-window 4x8 L6"},
+    {0, 8, 4, 6, "column-window 8x4 L6"},
+    {0, 8, 16, 6, "column-window 8x16 L6"},
+    {0, 16, 8, 6, "column-window 16x8 L6"},
+    {0, 8, 16, 4, "column-window 8x16 L4"},
+    {0, 8, 8, 8, "column-window 8x8 L8"},
+    {3, 8, 8, 2, "vector-lane 8x8 L2"},
+    {3, 8, 8, 4, "vector-lane 8x8 L4"},
+    {3, 8, 16, 2, "vector-lane 8x16 L2"},
 };
 
 int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
@@ -1262,6 +1634,23 @@ void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int
 }
 
 // loader-wave kernels: forward lookahead AF steps, adjoint AA steps
+template <int TI, int TJ, int L>
+void launch_vec(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
+                int s2, int s3, dim3 grid)
+{
+    const dim3 block(TI * TJ);
+    if (adjoint) {
+        if (s3 > 0)
+            hipLaunchKernelGGL((adj3d_vec<TI, TJ, L, true>), grid, block, 0, st, a, D, s1, s2);
+        else
+            hipLaunchKernelGGL((adj3d_vec<TI, TJ, L, false>), grid, block, 0, st, a, D, s1, s2);
+    } else if (uniform) {
+        hipLaunchKernelGGL((fwd3d_vec<TI, TJ, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
+    } else {
+        hipLaunchKernelGGL((fwd3d_vec<TI, TJ, L, false>), grid, block, 0, st, a, D, s1, s2, s3);
+    }
+}
+
 template <int TI, int TJ, int AF, int AA>
 void launch_dma(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
                 int s2, int s3, dim3 grid)
@@ -1305,6 +1694,17 @@ hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int ad
     case 3: launch_col<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 4: launch_col<8, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 5: launch_dma<8, 8, 12, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 6: launch_col<4, 4, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 7: launch_col<4, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 8: launch_col<4, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 9: launch_col<8, 4, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 10: launch_col<8, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 11: launch_col<16, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 12: launch_col<8, 16, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 13: launch_col<8, 8, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 14: launch_vec<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 15: launch_vec<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 16: launch_vec<8, 16, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

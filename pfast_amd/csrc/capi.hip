@@ -82,7 +82,7 @@ int default_ctx(pfast_ctx **out)
 }  // namespace
 
 namespace pfast {
-void time_begin(pfast_ctx *ctx, int kind)
+void time_begin(pfast_ctx *ctx, int kind, hipStream_t st)
 {
     if (!ctx->timing) return;
     if (ctx->evnext + 2 > ctx->evpool.size()) {
@@ -94,14 +94,14 @@ void time_begin(pfast_ctx *ctx, int kind)
         }
     }
     ctx->pending.push_back({kind, (int)ctx->evnext});
-    (void)hipEventRecord(ctx->evpool[ctx->evnext], ctx->stream);
+    (void)hipEventRecord(ctx->evpool[ctx->evnext], st ? st : ctx->stream);
     ctx->evnext += 2;
 }
 
-void time_end(pfast_ctx *ctx)
+void time_end(pfast_ctx *ctx, hipStream_t st)
 {
     if (!ctx->timing) return;
-    (void)hipEventRecord(ctx->evpool[ctx->pending.back().second + 1], ctx->stream);
+    (void)hipEventRecord(ctx->evpool[ctx->pending.back().second + 1], st ? st : ctx->stream);
 }
 
 // After a stream sync: fold the recorded pairs into the totals.
@@ -160,6 +160,11 @@ int pfast_ctx_destroy(pfast_ctx *ctx)
     for (auto &e : ctx->cache)
         if (e.second.first) (void)hipFree(e.second.first);
     for (auto e : ctx->evpool) (void)hipEventDestroy(e);
+    for (auto e : ctx->fork_join) (void)hipEventDestroy(e);
+    for (auto a : ctx->aux) {
+        (void)hipStreamSynchronize(a);
+        (void)hipStreamDestroy(a);
+    }
     (void)hipStreamDestroy(ctx->stream);
     if (ctx == g_default) g_default = nullptr;
     delete ctx;
@@ -186,6 +191,32 @@ const char *pfast_sweep_name(int variant)
 {
     if (variant < 0 || variant >= pfast::sweep_variant_count()) return nullptr;
     return pfast::sweep_variant(variant).name;
+}
+
+int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams)
+{
+    ARGCHK(ctx && nstreams >= 1 && nstreams <= 4, "nstreams must be 1..4");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    while ((int)ctx->aux.size() < nstreams - 1) {
+        hipStream_t a;
+        HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        ctx->aux.push_back(a);
+    }
+    while ((int)ctx->fork_join.size() < nstreams) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->fork_join.push_back(e);
+    }
+    ctx->nstreams = nstreams;
+    return PFAST_OK;
+}
+
+int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ctx->tile_skip = enable ? 1 : 0;
+    return PFAST_OK;
 }
 
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable)
@@ -557,17 +588,60 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
     for (int i = 0; i < s->nchunk; i++) act[i] = i;
     cyc.assign(s->nsrc, 0);
     int cycle = 0;
+    // tile skip records of this phase (column-window variants)
+    pfast::TileSkip skip{nullptr, nullptr, 0};
+    const SweepVariant &sv = pfast::sweep_variant(ctx->sweep);
+    if (ctx->tile_skip && sv.kind == 0) {
+        const int ntile = ((m->nrows + sv.ti - 1) / sv.ti) * ((m->ncols + sv.tj - 1) / sv.tj);
+        if (ntile != s->ntile) {
+            pfast::dfree(ctx, s->tmod);
+            pfast::dfree(ctx, s->tclean);
+            s->tmod = s->tclean = nullptr;
+            s->ntile = 0;
+            int rc;
+            if ((rc = pfast::dalloc(ctx, &s->tmod, (size_t)s->nchunk * ntile)) ||
+                (rc = pfast::dalloc(ctx, &s->tclean, (size_t)s->nchunk * ntile * 8)))
+                return rc;
+            s->ntile = ntile;
+        }
+        HIPCHK(hipMemsetAsync(s->tmod, 0, (size_t)s->nchunk * ntile * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(s->tclean, 0, (size_t)s->nchunk * ntile * 8 * sizeof(int), st));
+        skip.tmod = s->tmod;
+        skip.tclean = s->tclean;
+    }
     while (!act.empty() && cycle < MAXSwp) {
         cycle++;
-        HIPCHK(hipMemcpyAsync(s->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        // the active chunks, round-robin over the context's streams (one list each)
+        const int K = std::max(1, std::min(ctx->nstreams, (int)act.size()));
+        std::vector<int> lists;
+        std::vector<int> off(K + 1, 0);
+        for (int k = 0; k < K; k++) {
+            for (size_t i = k; i < act.size(); i += K) lists.push_back(act[i]);
+            off[k + 1] = (int)lists.size();
+        }
+        HIPCHK(hipMemcpyAsync(s->active, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
-        const int n = (int)act.size();
         long real = 0;
         for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
         ctx->units[adjoint ? 1 : 0] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
-        int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, s->active, n, s->changed)
-                         : pfast::issue_cycle3d(ctx, g, s->chunks, s->active, n, nullptr, 0, s->changed);
-        if (rc) return rc;
+        if (K > 1) {
+            HIPCHK(hipEventRecord(ctx->fork_join[0], st));
+            for (int k = 1; k < K; k++) HIPCHK(hipStreamWaitEvent(ctx->aux[k - 1], ctx->fork_join[0], 0));
+        }
+        skip.cycle = cycle - 1;
+        const pfast::TileSkip *sp = skip.tmod ? &skip : nullptr;
+        for (int k = 0; k < K; k++) {
+            hipStream_t sk = k ? ctx->aux[k - 1] : st;
+            const int *lk = s->active + off[k];
+            const int nk = off[k + 1] - off[k];
+            int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp)
+                             : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp);
+            if (rc) return rc;
+        }
+        for (int k = 1; k < K; k++) {
+            HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->aux[k - 1]));
+            HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
+        }
         HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         pfast::time_collect(ctx);
@@ -598,9 +672,10 @@ int diagonals3d(const pfast_ctx *ctx, const Grid3 &g)
 }
 
 int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1)
+                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
+                      hipStream_t st, const TileSkip *skip)
 {
-    hipStream_t st = ctx->stream;
+    if (!st) st = ctx->stream;
     const SweepVariant &sv = sweep_variant(ctx->sweep);
     const int nti = (g.nrows + sv.ti - 1) / sv.ti, ntj = (g.ncols + sv.tj - 1) / sv.tj;
     SlabArgs a{};
@@ -617,6 +692,10 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     a.wx = (float)(1.0 / (g.dx * g.dx));
     a.wy = (float)(1.0 / (g.dy * g.dy));
     a.wz = (float)(1.0 / (g.dz * g.dz));
+    const bool skipping = skip && sv.kind == 0;
+    a.tmod = skipping ? skip->tmod : nullptr;
+    a.tclean = skipping ? skip->tclean : nullptr;
+    a.o = o;
     SlabArgs af = a, aa = a;
     af.active = fwd;
     af.nact = nfwd;
@@ -625,28 +704,32 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     aa.nact = nadj;
     aa.nact_p8 = (nadj + 7) / 8 * 8;
     const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
+    const int nd = nti + ntj - 1;
     for (int D = D0; D < D1; D++) {
         const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
+        if (skipping) af.epoch = aa.epoch = 1 + ((skip->cycle * 8 + o) * nd + D);
         if (nfwd) {
-            time_begin(ctx, 0);
+            time_begin(ctx, 0, st);
             HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
-            time_end(ctx);
+            time_end(ctx, st);
         }
         if (nadj) {
-            time_begin(ctx, 1);
+            time_begin(ctx, 1, st);
             HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
-            time_end(ctx);
+            time_end(ctx, st);
         }
     }
     return PFAST_OK;
 }
 
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0, int o1)
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0, int o1,
+                  hipStream_t st, const TileSkip *skip)
 {
     const int nd = diagonals3d(ctx, g);
     for (int o = o0; o < o1; o++) {
-        int rc = issue_diagonals3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, nd);
+        int rc = issue_diagonals3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, nd, st,
+                                   skip);
         if (rc) return rc;
     }
     return PFAST_OK;
@@ -815,7 +898,8 @@ void shots3d_free(pfast_shots3d *s)
     s->m->nshots--;
     void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
                     s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
-                    s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active, s->chunks};
+                    s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active, s->chunks,
+                    s->tmod, s->tclean};
     for (void *p : ptrs) pfast::dfree(s->m->ctx, p);
     if (s->hchanged) (void)hipHostFree(s->hchanged);
     delete s;
@@ -833,8 +917,8 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     const int nrows = m->nrows, ncols = m->ncols, nz = m->nz;
     const long N = (long)nrows * ncols * nz;
     // the sweep kernels address one chunk field (C sources) with 32-bit byte offsets
-    ARGCHK(N * PFAST_CHUNK * 4 <= (1L << 30), "grid too large for the batched solver (a field "
-                                              "of one source chunk must stay within 1 GiB)");
+    ARGCHK(N * PFAST_CHUNK * 4 <= (1L << 29), "grid too large for the batched solver (a field "
+                                              "of one source chunk must stay within 512 MiB)");
     for (int i = 0; i < nsrc; i++) ARGCHK(srnum[i] >= 0, "negative srnum");
     auto *s = new pfast_shots3d();
     s->m = m;

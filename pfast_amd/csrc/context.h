@@ -36,6 +36,12 @@ struct pfast_ctx {
     // grow-only device scratch for the single-source forms
     std::map<std::string, std::pair<void *, size_t>> cache;
     DevicePool *pool = nullptr;  // set for the contexts of a request server
+    // batched 3D cycles: the active chunks split over `nstreams` streams (stream and
+    // aux[0..nstreams-2]) so that one stream's launch tails fill with another's work
+    int nstreams = 1;
+    int tile_skip = 1;  // exact tile skipping in the column-window sweeps
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> fork_join;  // [0] fork on stream, [1..] joins on aux
 };
 
 namespace pfast {
@@ -44,8 +50,8 @@ constexpr int kTimedKinds = 3;
 
 // An event pair around one launch when timing is enabled; folded into the
 // totals by time_collect after a stream synchronisation.
-void time_begin(pfast_ctx *ctx, int kind);
-void time_end(pfast_ctx *ctx);
+void time_begin(pfast_ctx *ctx, int kind, hipStream_t st = nullptr);  // nullptr: ctx->stream
+void time_end(pfast_ctx *ctx, hipStream_t st = nullptr);
 void time_collect(pfast_ctx *ctx);
 
 template <class T>

@@ -28,6 +28,8 @@ struct pfast_shots3d {
     int *srcnode = nullptr;  // [nchunk*C]
     float *T = nullptr, *L = nullptr;
     ChunkRef *chunks = nullptr;          // [nchunk] device table the sweep kernels read
+    int *tmod = nullptr, *tclean = nullptr;  // tile skip records, [nchunk][ntile](x8)
+    int ntile = 0;
     std::vector<ChunkRef> hchunks;       // host copy
     // receiver table (reference order) for picks and the single-source path
     long *rnode = nullptr;
@@ -70,11 +72,21 @@ inline Grid3 grid_of(const pfast_model3d *m)
 // anti-diagonal) on ctx's stream over the chunk ids fwd[0..nfwd) (forward sweeps)
 // and adj[0..nadj) (adjoint sweeps) of the chunk table `chunks`, all of grid g;
 // device arrays.  changed[id] collects, per chunk id, the sources that changed.
+// Exact tile skipping records of one phase (see SlabArgs) and the cycle number that
+// makes launch epochs increase; records are indexed by chunk id.
+struct TileSkip {
+    int *tmod, *tclean;
+    int cycle;
+};
+
+// st: the stream to issue on (nullptr: ctx->stream); skip: NULL for no skipping.
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0 = 0, int o1 = 8);
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0 = 0, int o1 = 8,
+                  hipStream_t st = nullptr, const TileSkip *skip = nullptr);
 // Tile anti-diagonals [D0, D1) of ordering o (same arguments otherwise).
 int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1);
+                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
+                      hipStream_t st = nullptr, const TileSkip *skip = nullptr);
 // Tile anti-diagonals of one ordering under ctx's sweep variant.
 int diagonals3d(const pfast_ctx *ctx, const Grid3 &g);
 

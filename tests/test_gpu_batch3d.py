@@ -133,6 +133,63 @@ def test_sweep_variants_bitwise_identical(seed, shape, spacing):
         assert np.array_equal(out[3], ref[3]) and out[4] == ref[4], name
 
 
+def test_stream_split_bitwise_identical():
+    """Splitting a cycle's chunks over 1-4 streams changes only which launch carries
+    a chunk, so every result is bit-identical; 0 and 5 streams are refused."""
+    import pfast_amd
+    c = cases.case3d(44, shape=(14, 23, 19), spacing=(25.0, 25.0, 25.0), nsrc=23, nrec_per=8)
+    ref = None
+    for k in (1, 2, 3, 4):
+        ctx = pfast_amd.Context(0)
+        ctx.set_streams(k)
+        m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+        sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+        sh.run(adjoint=True, MAXSwp=100)
+        g, E = sh.gradient()
+        out = (np.stack([sh.field(s) for s in range(23)]),
+               np.stack([sh.field(s, adjoint=True) for s in range(23)]), sh.cycles(), g, E)
+        sh.close()
+        m.close()
+        if k == 4:
+            for bad in (0, 5):
+                with pytest.raises(pfast_amd.PfastError):
+                    ctx.set_streams(bad)
+        ctx.close()
+        if ref is None:
+            ref = out
+            continue
+        assert all(np.array_equal(cases.bits(a), cases.bits(b)) for a, b in zip(out[:4], ref[:4])), k
+        assert out[4] == ref[4], k
+
+
+@pytest.mark.parametrize("seed,shape,spacing", [
+    (45, (21, 37, 29), (25.0, 25.0, 25.0)),
+    (46, (16, 25, 41), (12.5, 10.0, 8.0)),
+])
+def test_tile_skip_bitwise_identical(seed, shape, spacing):
+    """Exact tile skipping only leaves out sweeps that would change nothing: fields,
+    cycle counts, gradient and misfit equal those with every tile swept."""
+    import pfast_amd
+    c = cases.case3d(seed, shape=shape, spacing=spacing, nsrc=9, nrec_per=12)
+    outs = []
+    for skip in (False, True):
+        ctx = pfast_amd.Context(0)
+        ctx.set_tile_skip(skip)
+        m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+        sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+        sh.run(adjoint=True, MAXSwp=100)
+        g, E = sh.gradient()
+        outs.append((np.stack([sh.field(s) for s in range(9)]),
+                     np.stack([sh.field(s, adjoint=True) for s in range(9)]), sh.cycles(), g, E))
+        sh.close()
+        m.close()
+        ctx.close()
+    a, b = outs
+    print(f"cycles {a[2].tolist()}", flush=True)
+    assert all(np.array_equal(cases.bits(x), cases.bits(y)) for x, y in zip(a[:4], b[:4]))
+    assert a[4] == b[4]
+
+
 def test_sweep_variant_rejects_unknown():
     import pfast_amd
     ctx = pfast_amd.Context(0)

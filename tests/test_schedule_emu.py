@@ -49,7 +49,9 @@ SHAPES = [((11, 19, 13), (8, 8, 2)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 
           ((7, 5, 23), (8, 4, 1)), ((12, 16, 9), (8, 8, 4))]
 # column window: the batch depth L is even (static step parity inside a batch)
 COL_SHAPES = [((11, 19, 13), (8, 8, 4)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 4, 4)),
-              ((7, 5, 23), (8, 4, 6)), ((12, 16, 9), (8, 8, 4)), ((3, 17, 9), (8, 8, 4))]
+              ((7, 5, 23), (8, 4, 6)), ((12, 16, 9), (8, 8, 4)), ((3, 17, 9), (8, 8, 4)),
+              ((10, 14, 11), (4, 4, 6)), ((8, 13, 19), (4, 8, 6)), ((9, 21, 35), (8, 16, 6)),
+              ((7, 37, 18), (16, 8, 4))]
 CASES = [(sh, t, "slab") for sh, t in SHAPES] + [(sh, t, "col") for sh, t in COL_SHAPES]
 
 
