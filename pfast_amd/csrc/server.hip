@@ -96,6 +96,12 @@ struct Group {
     bool inflight = false;        // a cycle is in progress
     int o = 0, D = 0;             // next slice: ordering, first anti-diagonal
     int issued = 0, done = 0, nfwd = 0, nadj = 0;
+    // exact tile skipping: records by chunk id (cap x ntile), zeroed when a chunk
+    // starts a phase; epochs from the group's cycle counter
+    int *tmod = nullptr, *tclean = nullptr;
+    int ntile = 0;
+    long cycle_no = 0;
+    std::vector<int> reset;  // chunk ids whose records are zeroed before the next cycle
     std::vector<int> cyc_ids;  // chunk ids of the cycle in flight
     std::set<long> cyc_reqs;   // requests of the cycle in flight
 };
@@ -170,6 +176,21 @@ int grow_group(Group &G, int need)
         pfast::set_error("allocation of a group's chunk tables failed");
         return PFAST_ERR_NOMEM;
     }
+    int *tm = nullptr, *tc = nullptr;
+    if (G.ntile &&
+        (hipMalloc(&tm, (size_t)cap * G.ntile * sizeof(int)) != hipSuccess ||
+         hipMalloc(&tc, (size_t)cap * G.ntile * 8 * sizeof(int)) != hipSuccess ||
+         hipMemsetAsync(tm, 0, (size_t)cap * G.ntile * sizeof(int), G.ctx->stream) != hipSuccess ||
+         hipMemsetAsync(tc, 0, (size_t)cap * G.ntile * 8 * sizeof(int), G.ctx->stream) != hipSuccess)) {
+        (void)hipGetLastError();
+        for (void *q : {(void *)dt, (void *)da, (void *)dc, (void *)tm, (void *)tc})
+            if (q) (void)hipFree(q);
+        (void)hipHostFree(ht);
+        (void)hipHostFree(ha);
+        (void)hipHostFree(hc);
+        pfast::set_error("allocation of a group's tile records failed");
+        return PFAST_ERR_NOMEM;
+    }
     if (G.dtable) {
         (void)hipFree(G.dtable);
         (void)hipFree(G.dact);
@@ -178,6 +199,11 @@ int grow_group(Group &G, int need)
         (void)hipHostFree(G.hact);
         (void)hipHostFree(G.hchanged);
     }
+    if (G.tmod) (void)hipFree(G.tmod);
+    if (G.tclean) (void)hipFree(G.tclean);
+    G.tmod = tm;
+    G.tclean = tc;
+    G.reset.clear();  // fresh records are zero
     G.dtable = dt;
     G.htable = ht;
     G.dact = da;
@@ -204,6 +230,11 @@ int group_of(pfast_server *S, const pfast::Grid3 &g, int *out)
     int rc = pfast_ctx_create(S->device, &G->ctx);
     if (rc) return rc;
     G->ctx->pool = &S->pool;
+    {
+        const SweepVariant &sv = pfast::sweep_variant(G->ctx->sweep);
+        if (G->ctx->tile_skip && sv.kind == 0)
+            G->ntile = ((g.nrows + sv.ti - 1) / sv.ti) * ((g.ncols + sv.tj - 1) / sv.tj);
+    }
     for (auto &e : G->ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
             for (auto &f : G->ev)
@@ -264,6 +295,7 @@ int admit3d(pfast_server *S, SRequest &R, SModel &M)
         G.owner[id] = R.id;
         G.lchunk[id] = c;
         R.gid[c] = id;
+        G.reset.push_back(id);
     }
     G.dirty = true;
     R.live.assign(nch, 1);
@@ -328,8 +360,10 @@ int pump_cycle(Group &G)
     const int nd = pfast::diagonals3d(G.ctx, G.g);
     while (G.o < 8 && G.issued - G.done < kQueued) {
         const int D1 = std::min(G.D + kSlice, nd);
+        const pfast::TileSkip skip{G.tmod, G.tclean, (int)G.cycle_no};
         int rc = pfast::issue_diagonals3d(G.ctx, G.g, G.dtable, G.dact, G.nfwd, G.dact + G.nfwd,
-                                          G.nadj, G.dchanged, G.o, G.D, D1);
+                                          G.nadj, G.dchanged, G.o, G.D, D1, nullptr,
+                                          G.tmod ? &skip : nullptr);
         if (rc) return rc;
         G.D = D1;
         if (G.D == nd) {
@@ -371,6 +405,19 @@ int issue_cycle(pfast_server *S, Group &G)
     }
     if (fwd.empty() && adj.empty()) return PFAST_OK;
     for (long id : G.cyc_reqs) S->reqs[id].cycle++;
+    if (G.tmod) {
+        if (G.cycle_no > (1L << 20)) {  // keep epochs far below INT_MAX: forget all records
+            HIPCHK(hipMemsetAsync(G.tmod, 0, (size_t)G.cap * G.ntile * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tclean, 0, (size_t)G.cap * G.ntile * 8 * sizeof(int), st));
+            G.cycle_no = 0;
+            G.reset.clear();
+        }
+        for (int id : G.reset) {
+            HIPCHK(hipMemsetAsync(G.tmod + (size_t)id * G.ntile, 0, G.ntile * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tclean + (size_t)id * G.ntile * 8, 0, G.ntile * 8 * sizeof(int), st));
+        }
+        G.reset.clear();
+    }
     std::copy(fwd.begin(), fwd.end(), G.hact);
     std::copy(adj.begin(), adj.end(), G.hact + fwd.size());
     const int na = (int)(fwd.size() + adj.size());
@@ -379,6 +426,7 @@ int issue_cycle(pfast_server *S, Group &G)
     G.nfwd = (int)fwd.size();
     G.nadj = (int)adj.size();
     G.o = G.D = G.issued = G.done = 0;
+    G.cycle_no++;
     G.cyc_ids = fwd;
     G.cyc_ids.insert(G.cyc_ids.end(), adj.begin(), adj.end());
     G.inflight = true;
@@ -427,6 +475,7 @@ void finish_cycle(pfast_server *S, Group &G, int *ndone)
             R.cycle = 0;
             R.cyc_a.assign(R.nsrc, 0);
             R.live.assign(R.live.size(), 1);
+            G.reset.insert(G.reset.end(), R.gid.begin(), R.gid.end());
         } else {
             free_chunks(G, R);
             int rc = pfast::adj_end3d(R.s3);
@@ -663,6 +712,8 @@ int pfast_server_destroy(pfast_server *S)
             (void)hipHostFree(G->hact);
             (void)hipHostFree(G->hchanged);
         }
+        if (G->tmod) (void)hipFree(G->tmod);
+        if (G->tclean) (void)hipFree(G->tclean);
         for (auto &e : G->ev)
             if (e) (void)hipEventDestroy(e);
         pfast_ctx_destroy(G->ctx);
