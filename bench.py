@@ -95,7 +95,7 @@ def cpu_baseline(cfg, nthreads, nsamp, maxswp):
 def measured_traffic(kernel, sweep_name, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (profiles/traffic.json, copy-calibrated), when they were taken
-    on this same workload and sweep variant; else None."""
+    on this same workload, sweep variant and tile-skip setting with one stream; else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
@@ -103,7 +103,8 @@ def measured_traffic(kernel, sweep_name, args):
     except (OSError, ValueError):
         return None
     if (t.get("sweep") != sweep_name or t.get("nsrc") != args.nsrc or t.get("nrec") != args.nrec
-            or t.get("grid") != [101, 201, 201]):
+            or t.get("grid") != [101, 201, 201]
+            or t.get("tile_skip", False) != (not args.no_tile_skip) or args.streams != 1):
         return None
     return t.get("bytes_per_launch", {}).get(kernel)
 
