@@ -13,6 +13,8 @@ the CPU, one MPI rank) and keeps, in driver2d.npz / driver3d.npz:
   E, Et, Er   -- the misfit per iteration (the reference's _GlobalErr files)
   steps       -- the accepted step size per iteration (its log)
   trials      -- every line-search trial (alpha0, alpha1, phi0, phi1, dphi0, dphi1)
+  res_*       -- 3D only: a run resumed from the model after iteration 2 (its input
+                 files and the reference's resumed misfits, steps and trials)
   E_ens, steps_ens -- the same run from NENS starting models perturbed by 1e-5
                  relative (seeded normal noise), NaN-padded: the reference's own
                  sensitivity, which bounds how closely any other arithmetic can
@@ -116,6 +118,24 @@ def write3d(d, c, lsrc=2):
                   vpfile="final.vp", nux=50, nuy=50, nuz=25, schemetag=3, lsrc=lsrc)
 
 
+def write_resume3d(d, c, res):
+    """Files of a resumed 3D run: vp0file m.vp2 (last digit = iterations done), the
+    gradients before regularisation of iterations 2 and 3, global_E.tmp, alpha1.tmp."""
+    shape = (c["nz"], c["nr"], c["nc"])
+    F.write_bin(os.path.join(d, "m.vp2"), res["vp2"].reshape(shape))
+    F.write_bin(os.path.join(d, "m_bf_regu_2.vp"), res["bf2"].reshape(shape))
+    F.write_bin(os.path.join(d, "m_bf_regu_3.vp"), res["bf3"].reshape(shape))
+    F.write_bin(os.path.join(d, "m.src"), c["srcA"])
+    F.write_bin(os.path.join(d, "m.rec"), c["recA"])
+    with open(os.path.join(d, "global_E.tmp"), "w") as fh:
+        fh.write("".join(f"{e:.17g}\n" for e in res["gE"]))
+    with open(os.path.join(d, "alpha1.tmp"), "w") as fh:
+        fh.write(f"{float(res['a1'][0]):.17g}\n")
+    F.write_inp3d(os.path.join(d, "m.inp"), "m.vp2", "m.src", c["srcA"].shape, "m.rec",
+                  c["recA"].shape, c["nr"], c["nc"], c["nz"], c["h"], c["h"], c["h"],
+                  vpfile="final.vp", nux=50, nuy=50, nuz=25, schemetag=3, lsrc=2)
+
+
 def steps_from_log(log):
     return [float(v) for v in re.findall(r"Stepsize=([-0-9.eE+]+)", log)]
 
@@ -183,6 +203,21 @@ def main():
                     vpF=F.read_bin(os.path.join(d, "final.vp"), shape))
         out3["E_ens"], out3["steps_ens"] = ensemble(c, write3d, os.path.join(REFBIN, "T_PFAST3D"),
                                                     ["50"], "m_GlobalErr.txt")
+        # resume from the model after iteration 2 (the reference's vp0file-digit convention):
+        # its inputs from this run, and the reference's resumed trajectory
+        res = dict(vp2=F.read_bin(os.path.join(d, "m.vp2"), shape),
+                   bf2=F.read_bin(os.path.join(d, "m_bf_regu_2.vp"), shape),
+                   bf3=F.read_bin(os.path.join(d, "m_bf_regu_3.vp"), shape),
+                   gE=E[:3].copy(), a1=np.array([out3["steps"][1]]))
+        with tempfile.TemporaryDirectory() as d2:
+            write_resume3d(d2, c, res)
+            rc, log = run(os.path.join(REFBIN, "T_PFAST3D"), d2, ["50"], 600)
+            assert rc == 0, log[-2000:]
+            out3.update({"res_" + k: v for k, v in res.items()})
+            out3["res_E"] = np.loadtxt(os.path.join(d2, "m_GlobalErr.txt"), ndmin=1)
+            out3["res_steps"] = np.array(steps_from_log(log))
+            out3["res_trials"] = trials_from_log(log)
+        print(f"3D resume: E {out3['res_E']}, first trial {out3['res_trials'][0]}")
         print(f"3D: {len(E)} iterations, E {E[0]:.6g} -> {E[-1]:.6g}, steps {out3['steps'][:4]}")
         np.savez_compressed(os.path.join(HERE, "driver3d.npz"), **out3)
 

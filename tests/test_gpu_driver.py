@@ -216,3 +216,45 @@ def test_rt2d_forward_mode_writes_reference_picks(tmp_path):
     eu = np.abs(Tr[:, 2] - rows[:, 6]).max() / rows[:, 6].max()
     print(f"forward picks: transmission err {et:.2e}, reflector-2 err {eu:.2e}", flush=True)
     assert et <= 1e-5 and eu <= 1e-5
+
+
+def test_t3d_driver_resume_matches_reference(tmp_path):
+    """The reference's resume convention (src/T_PFAST3D.cpp:141-147,345-384): vp0file
+    m.vp2 ends in the number of finished iterations; the run reloads the gradients
+    m_bf_regu_2.vp / m_bf_regu_3.vp, global_E.tmp and alpha1.tmp and continues at
+    iteration 3.  Its first trial comes from those files and one evaluation, so it
+    must match the reference's resumed run (fixture res_*) closely."""
+    from pfast_amd import formats as F
+    g = np.load(os.path.join(GOLD, "driver3d.npz"))
+    nr, nc, nz = (int(v) for v in g["grid"])
+    h = float(g["spacing"][0])
+    shape = (nz, nr, nc)
+    d = str(tmp_path)
+    F.write_bin(os.path.join(d, "m.vp2"), g["res_vp2"].reshape(shape))
+    F.write_bin(os.path.join(d, "m_bf_regu_2.vp"), g["res_bf2"].reshape(shape))
+    F.write_bin(os.path.join(d, "m_bf_regu_3.vp"), g["res_bf3"].reshape(shape))
+    F.write_bin(os.path.join(d, "m.src"), g["srcA"])
+    F.write_bin(os.path.join(d, "m.rec"), g["recA"])
+    with open(os.path.join(d, "global_E.tmp"), "w") as fh:
+        fh.write("".join(f"{e:.17g}\n" for e in g["res_gE"]))
+    with open(os.path.join(d, "alpha1.tmp"), "w") as fh:
+        fh.write(f"{float(g['res_a1'][0]):.17g}\n")
+    F.write_inp3d(os.path.join(d, "m.inp"), "m.vp2", "m.src", g["srcA"].shape, "m.rec",
+                  g["recA"].shape, nr, nc, nz, h, h, h, vpfile="final.vp", nux=50, nuy=50,
+                  nuz=25, schemetag=3, lsrc=2)
+    log = _run("pfast_t3d", d, ["50"])
+    assert "Iterating No. 2 resumed" in log
+    E = np.loadtxt(os.path.join(d, "m_GlobalErr.txt"), ndmin=1)
+    ref = g["res_E"]
+    t = _trials(log)[0][0]
+    r = g["res_trials"][0]
+    print("E ours", E.tolist(), "\nE ref ", ref.tolist(), "\nfirst trial ours", t.tolist(),
+          "\nfirst trial ref ", r.tolist(), flush=True)
+    assert len(E) == len(ref)
+    assert np.all(np.abs(E[:3] - ref[:3]) <= 1e-5 * ref[:3])      # carried over from the files
+    assert np.all(np.abs(E[3:] - ref[3:]) <= 1e-4 * ref[3:])      # one GPU evaluation each
+    assert t[1] == r[1] and _rel(t[2], r[2]) <= 1e-5               # trial step, phi0 from files
+    assert _rel(t[4], r[4]) <= 1e-4                                # phi0' from the file gradients
+    assert _rel(t[3], r[3]) <= 1e-4                                # misfit at the trial model
+    for k in range(3, len(E)):
+        assert os.path.exists(os.path.join(d, f"m.vp{k}"))
