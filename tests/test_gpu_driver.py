@@ -258,3 +258,55 @@ def test_t3d_driver_resume_matches_reference(tmp_path):
     assert _rel(t[3], r[3]) <= 1e-4                                # misfit at the trial model
     for k in range(3, len(E)):
         assert os.path.exists(os.path.join(d, f"m.vp{k}"))
+
+
+def _residuals(log):
+    import re
+    return np.array([[float(x) for x in m] for m in re.findall(
+        r"Average Time Residual \(TR\): ([-0-9.eE+]+) ms\s+\(T\): ([-0-9.eE+]+) ms\s+"
+        r"\(R\): ([-0-9.eE+]+) ms", log)])
+
+
+@pytest.mark.parametrize("tag", [3, 1, 2])
+def test_rt2d_example_inversion_matches_reference_logs(tmp_path, tag):
+    """The reference's shipped 2D inversion example (tests/golden/example2d_inv.npz,
+    written by gen_example2d_inv.py from models/RT_SynMod2D_* and the logs
+    par/RT_SynMod2D_Inv_{RT,Trans,Ref}.out): 160 x 600, 56 sources x 600 receivers, two
+    reflectors, LM 50, RTtag 3 / 1 / 2.  Anchors at the logs' 6 digits: the residuals of
+    iterations 1 and 2 (1e-4, 1e-3), and iteration 1's line search: the trial step
+    500 / max|Dj| (1e-3), phi0 and phi0' (1e-4, 1e-3), the misfit and derivative at
+    both trials and the second trial step (1e-2), the accepted step (1e-2)."""
+    from pfast_amd import formats as F
+    e = np.load(os.path.join(GOLD, "example2d_inv.npz"))
+    nr, nc = (int(v) for v in e["grid"])
+    dx, dz = (float(v) for v in e["spacing"])
+    d = str(tmp_path)
+    n = len(e["tt"])
+    rec = np.concatenate([np.tile(e["geom"], (n // len(e["geom"]), 1)),
+                          e["tt"].astype(np.float64)[:, None], np.zeros((n, 1)),
+                          e["tr2"].astype(np.float64)[:, None]], 1)
+    F.write_bin(os.path.join(d, "m.vp"), e["vp"].astype(np.float64))
+    F.write_bin(os.path.join(d, "m.src"), e["src"])
+    F.write_bin(os.path.join(d, "m.rec"), rec)
+    F.write_bin(os.path.join(d, "m.ref"), e["ref"])
+    F.write_bin(os.path.join(d, "m.wr"), np.full((nr, nc), float(e["wr"][0])))
+    F.write_inp2d(os.path.join(d, "m.inp"), "m.vp", "m.src", e["src"].shape, "m.rec", rec.shape,
+                  "m.ref", e["ref"].shape, nr, nc, dx, dz, vpfile="final.vp",
+                  nux=float(e["nu"][0]), nuz=float(e["nu"][1]), twfile="m.wr", schemetag=3, lsrc=2)
+    log = _run("pfast_rt2d", d, ["50", str(tag), "--max-iter", "2", "--no-shot-gather"])
+    res = _residuals(log)
+    t = _trials(log)[0]
+    steps = _steps(log)
+    rres, rt, rstep = e[f"res{tag}"], e[f"trials{tag}"], e[f"step{tag}"]
+    print(f"RTtag {tag}\nresiduals ours {res[:2].tolist()}\nresiduals ref  {rres.tolist()}\n"
+          f"trials ours {t.tolist()}\ntrials ref  {rt.tolist()}\nstep ours {steps[:1]} ref {rstep}",
+          flush=True)
+    assert np.all(np.abs(res[0] - rres[0]) <= 1e-4 * rres[0])
+    assert len(t) >= 2
+    assert _rel(t[0, 1], rt[0, 1]) <= 1e-3
+    assert _rel(t[0, 2], rt[0, 2]) <= 1e-4 and _rel(t[0, 4], rt[0, 4]) <= 1e-3
+    for k in range(2):
+        assert _rel(t[k, 3], rt[k, 3]) <= 1e-2 and _rel(t[k, 5], rt[k, 5]) <= 1e-2
+    assert _rel(t[1, 1], rt[1, 1]) <= 1e-2
+    assert _rel(steps[0], rstep[0]) <= 1e-2
+    assert np.all(np.abs(res[1] - rres[1]) <= 1e-3 * rres[1])
