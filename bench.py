@@ -304,7 +304,11 @@ def main():
     model.close()
     ctx.close()
     if not args.no_serve:
-        out["serving"] = serving(args)
+        try:
+            out["serving"] = serving(args)
+        except Exception as exc:  # the headline line must still be printed
+            log(f"serving measurement failed: {exc!r}")
+            out["serving"] = {"error": repr(exc)}
     print(json.dumps(out), flush=True)
 
 
