@@ -23,10 +23,13 @@ struct SlabArgs {
     int nrows, ncols, nz, nti, ntj;
     float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
     // Exact tile skipping (column-window kernels; NULL: off).  Per (chunk, tile):
-    // tmod = epoch of the last sweep that changed a node of the tile, tclean[o] =
-    // epoch of the last sweep in ordering o that changed nothing.  A tile whose
-    // clean sweep in this ordering is newer than every change of it and of its four
-    // neighbour tiles would see the same inputs again and change nothing: skipped.
+    // tmod[5] = epochs of the last sweeps that changed a node of the tile ([0]) and
+    // of its boundary faces x-, x+, y-, y+ ([1..4]: column j = 0, column j = last,
+    // row i = 0, row i = last), tclean[o] = epoch of the last sweep in ordering o
+    // that changed nothing.  A tile reads from a neighbour only the face that borders
+    // it, so a tile whose clean sweep in this ordering is newer than every change of
+    // itself and of those four neighbour faces would see the same inputs again and
+    // change nothing: skipped.
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
 };

@@ -514,7 +514,9 @@ __device__ __forceinline__ int skip_index(const SlabArgs &a, const TileCtx &t)
     return a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
 }
 
-// The sweep of tile ix in this ordering would change nothing (see SlabArgs).
+// The sweep of tile ix in this ordering would change nothing (see SlabArgs): its
+// clean sweep is newer than every change of the tile and of the neighbour faces
+// it reads (left tile's x+ face, right's x-, upper's y+, lower's y-).
 template <int TI, int TJ>
 __device__ __forceinline__ bool skip_tile(const SlabArgs &a, const TileCtx &t, int ix)
 {
@@ -522,24 +524,41 @@ __device__ __forceinline__ bool skip_tile(const SlabArgs &a, const TileCtx &t, i
     const int cl = a.tclean[ix * 8 + a.o];
     if (cl <= 0) return false;
     const int ti = t.i0 / TI, tj = t.j0 / TJ;
-    int m = a.tmod[ix];
-    if (ti > 0) m = max(m, a.tmod[ix - a.ntj]);
-    if (ti < a.nti - 1) m = max(m, a.tmod[ix + a.ntj]);
-    if (tj > 0) m = max(m, a.tmod[ix - 1]);
-    if (tj < a.ntj - 1) m = max(m, a.tmod[ix + 1]);
+    const int *M = a.tmod;
+    int m = M[ix * 5];
+    if (tj > 0) m = max(m, M[(ix - 1) * 5 + 2]);
+    if (tj < a.ntj - 1) m = max(m, M[(ix + 1) * 5 + 1]);
+    if (ti > 0) m = max(m, M[(ix - a.ntj) * 5 + 4]);
+    if (ti < a.nti - 1) m = max(m, M[(ix + a.ntj) * 5 + 3]);
     return m < cl;
 }
 
-// Record the outcome of this tile sweep: any wave of the block changed a node.
-__device__ __forceinline__ void record_tile(const SlabArgs &a, int ix, unsigned long long wave_vote,
-                                            int *any)
+// Faces of the tile that this thread's column lies on (bit 0 x-, 1 x+, 2 y-, 3 y+).
+__device__ __forceinline__ unsigned col_faces(bool valid, int li, int lj, int TIe, int TJe)
+{
+    if (!valid) return 0u;
+    return (lj == 0 ? 1u : 0u) | (lj == TJe - 1 ? 2u : 0u) | (li == 0 ? 4u : 0u) |
+           (li == TIe - 1 ? 8u : 0u);
+}
+
+// Record the outcome of this tile sweep: `what` = bit 4 (some node changed) | the
+// faces on which a node changed, OR-ed over the block.
+__device__ __forceinline__ void record_tile(const SlabArgs &a, int ix, unsigned what, int *any)
 {
     if (ix < 0) return;
-    if ((threadIdx.x & 63) == 0 && wave_vote) atomicOr(any, 1);
+    unsigned w = 0;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        if (__ballot((what >> b) & 1u)) w |= 1u << b;
+    if ((threadIdx.x & 63) == 0 && w) atomicOr(any, (int)w);
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (*any) {
-            a.tmod[ix] = a.epoch;
+        const int v = *any;
+        if (v) {
+            a.tmod[ix * 5] = a.epoch;
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                if ((v >> f) & 1) a.tmod[ix * 5 + 1 + f] = a.epoch;
             a.tclean[ix * 8 + a.o] = 0;
         } else {
             a.tclean[ix * 8 + a.o] = a.epoch;
@@ -670,7 +689,8 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     }
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-    record_tile(a, six, bm, &any_change);
+    record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
+                &any_change);
 }
 
 // Adjoint, column window: lambda and traveltime travel together as float2 (the
@@ -803,7 +823,8 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     }
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-    record_tile(a, six, bm, &any_change);
+    record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
+                &any_change);
 }
 
 // ============================================================ vector-lane sweeps

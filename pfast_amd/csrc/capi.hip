@@ -599,12 +599,12 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
             s->tmod = s->tclean = nullptr;
             s->ntile = 0;
             int rc;
-            if ((rc = pfast::dalloc(ctx, &s->tmod, (size_t)s->nchunk * ntile)) ||
+            if ((rc = pfast::dalloc(ctx, &s->tmod, (size_t)s->nchunk * ntile * 5)) ||
                 (rc = pfast::dalloc(ctx, &s->tclean, (size_t)s->nchunk * ntile * 8)))
                 return rc;
             s->ntile = ntile;
         }
-        HIPCHK(hipMemsetAsync(s->tmod, 0, (size_t)s->nchunk * ntile * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(s->tmod, 0, (size_t)s->nchunk * ntile * 5 * sizeof(int), st));
         HIPCHK(hipMemsetAsync(s->tclean, 0, (size_t)s->nchunk * ntile * 8 * sizeof(int), st));
         skip.tmod = s->tmod;
         skip.tclean = s->tclean;

@@ -178,9 +178,9 @@ int grow_group(Group &G, int need)
     }
     int *tm = nullptr, *tc = nullptr;
     if (G.ntile &&
-        (hipMalloc(&tm, (size_t)cap * G.ntile * sizeof(int)) != hipSuccess ||
+        (hipMalloc(&tm, (size_t)cap * G.ntile * 5 * sizeof(int)) != hipSuccess ||
          hipMalloc(&tc, (size_t)cap * G.ntile * 8 * sizeof(int)) != hipSuccess ||
-         hipMemsetAsync(tm, 0, (size_t)cap * G.ntile * sizeof(int), G.ctx->stream) != hipSuccess ||
+         hipMemsetAsync(tm, 0, (size_t)cap * G.ntile * 5 * sizeof(int), G.ctx->stream) != hipSuccess ||
          hipMemsetAsync(tc, 0, (size_t)cap * G.ntile * 8 * sizeof(int), G.ctx->stream) != hipSuccess)) {
         (void)hipGetLastError();
         for (void *q : {(void *)dt, (void *)da, (void *)dc, (void *)tm, (void *)tc})
@@ -407,13 +407,13 @@ int issue_cycle(pfast_server *S, Group &G)
     for (long id : G.cyc_reqs) S->reqs[id].cycle++;
     if (G.tmod) {
         if (G.cycle_no > (1L << 20)) {  // keep epochs far below INT_MAX: forget all records
-            HIPCHK(hipMemsetAsync(G.tmod, 0, (size_t)G.cap * G.ntile * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tmod, 0, (size_t)G.cap * G.ntile * 5 * sizeof(int), st));
             HIPCHK(hipMemsetAsync(G.tclean, 0, (size_t)G.cap * G.ntile * 8 * sizeof(int), st));
             G.cycle_no = 0;
             G.reset.clear();
         }
         for (int id : G.reset) {
-            HIPCHK(hipMemsetAsync(G.tmod + (size_t)id * G.ntile, 0, G.ntile * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tmod + (size_t)id * G.ntile * 5, 0, G.ntile * 5 * sizeof(int), st));
             HIPCHK(hipMemsetAsync(G.tclean + (size_t)id * G.ntile * 8, 0, G.ntile * 8 * sizeof(int), st));
         }
         G.reset.clear();
