@@ -127,6 +127,8 @@ int main(int argc, char **argv)
 
     pfast_ctx *ctx = nullptr;
     check(pfast_ctx_create(0, &ctx), "pfast_ctx_create");
+    // a cycle's source chunks split over 3 streams: same results, more launches in flight
+    check(pfast_ctx_set_streams(ctx, 3), "pfast_ctx_set_streams");
     pfast_model3d *model = nullptr;
     check(pfast_model3d_create(ctx, vp.data(), NX, NY, NZ, dx, dy, dz, &model), "pfast_model3d_create");
     pfast_shots3d *shots = nullptr;
