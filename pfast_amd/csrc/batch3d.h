@@ -12,6 +12,8 @@ struct ChunkRef {
     float *field[2];           // [N][C]: [0] traveltimes (negative marks receiver
                                // nodes during the adjoint), [1] lambda
     const float *slow;         // [N] the model's cell time h/v (uniform) or slowness 1/v
+    const float *slow8;        // the same values 8x8-tile-major: [nz][nti8][ntj8][8][8]
+                               // (tile (ti,tj) row li column lj; edge tiles padded)
     int srcnode[PFAST_CHUNK];  // linear source node, -1 for padding sources
 };
 
@@ -64,4 +66,8 @@ hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, in
                                double *out);
 hipError_t launch_cell_time(hipStream_t st, const double *vp, float *f, long N, double h,
                             int uniform);
+// Elements of the 8x8-tile-major copy of a cell-time array, and the copy itself.
+long tile8_size(int nrows, int ncols, int nz);
+hipError_t launch_tile8_copy(hipStream_t st, const float *f, float *ft, int nrows, int ncols,
+                             int nz);
 }  // namespace pfast

@@ -448,6 +448,25 @@ __device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
 // so that the compiler's own wait insertion knows nothing is outstanding after it.
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Lane Q of this lane's quad (DPP quad_perm [Q,Q,Q,Q]).  The intrinsic is convergent,
+// so it stays where every lane of the quad is active.
+template <int Q>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_bcast(float v, int q)
+{
+    switch (q & 3) {  // q is a constant after unrolling
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+    }
+}
+
 
 // Per-thread constants of a column-window tile task.  Level lev (sweep order) of
 // the thread's column sits at byte offset oI + lev*step.
@@ -571,6 +590,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
+    static_assert(C == 4, "a node's sources fill one DPP quad");
     // per step parity: [m-1 | m+1][thread], then an INF slot (domain-edge neighbour)
     constexpr int PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
@@ -587,7 +607,12 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     const unsigned plane = (unsigned)a.nrows * a.ncols;
     const unsigned fbytes = 4u * plane * (unsigned)nz * C;
     const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
-    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
+    // 8x8 tiles read the tile-major cell times: a tile's level is 256 contiguous bytes
+    // (two cache lines) instead of eight 32-byte pieces of grid rows
+    constexpr bool F8 = TI == 8 && TJ == 8;
+    const unsigned tplane = (unsigned)a.nti * a.ntj * 64u;
+    const rsrc_t RS = F8 ? make_rsrc(a.chunks[t.chunk].slow8, 4u * tplane * (unsigned)nz)
+                         : make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
     const float INF = PFAST_INF_F;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = INF;
@@ -604,9 +629,24 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
             psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
         }
     }
-    const unsigned st = l.step, stF = l.stepF;
+    const unsigned st = l.step;
+    unsigned oF = l.oF, stF = l.stepF;
+    if (F8) {
+        const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
+        const unsigned tix = (unsigned)(t.i0 / TI) * a.ntj + (unsigned)(t.j0 / TJ);
+        oF = 4u * (k0 * tplane + tix * 64u + (unsigned)(l.li * TJ + l.lj));
+        stF = s3 > 0 ? 4u * tplane : 0u - 4u * tplane;
+    }
+    // Vector-memory instructions are what the sweep's throughput is bound by (each
+    // costs the address unit the same whatever its lanes hold), so: halo loads run
+    // only in the lanes that use them (a wave without such a lane skips them), and
+    // the cell time, one value per node, is loaded for four levels at once, lane c
+    // of the node's four lanes taking level 4g + c; step u broadcasts it by DPP.
+    constexpr int FQ = (L + 3) / 4;
+    const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
+    const unsigned oFc = oF + (unsigned)l.c * stF;
     // queue of one batch: own column two levels ahead (m+2, INF outside the column),
-    // halos and cell time at level m itself
+    // halos at level m itself, cell time of levels m .. m+4*FQ-1
     auto load_batch = [&](int pb, float *ow, float *hx, float *hy, float *fv) {
         const int mb = pb - l.pipj;
         const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
@@ -614,10 +654,23 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
         for (int u = 0; u < L; u++) {
             const float o = bload(RF, l.oI + dm + (unsigned)(u + 2) * st);
             ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF;
-            hx[u] = bload(RF, l.oHX + dm + (unsigned)u * st);
-            hy[u] = bload(RF, l.oHY + dm + (unsigned)u * st);
-            fv[u] = bload(RS, l.oF + dmf + (unsigned)u * stF);
         }
+        if (hxuse) {
+#pragma unroll
+            for (int u = 0; u < L; u++) hx[u] = bload(RF, l.oHX + dm + (unsigned)u * st);
+        } else {
+#pragma unroll
+            for (int u = 0; u < L; u++) hx[u] = INF;
+        }
+        if (hyuse) {
+#pragma unroll
+            for (int u = 0; u < L; u++) hy[u] = bload(RF, l.oHY + dm + (unsigned)u * st);
+        } else {
+#pragma unroll
+            for (int u = 0; u < L; u++) hy[u] = INF;
+        }
+#pragma unroll
+        for (int g = 0; g < FQ; g++) fv[g] = bload(RS, oFc + dmf + (unsigned)(4 * g) * stF);
     };
     // window at step 0 (level m0 = -pipj); level m0-1 is outside the column
     const int m0 = -l.pipj;
@@ -658,12 +711,13 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
             const float xa = fminf(l.uxm ? q[1][u] : pxm, l.uxp ? q[1][u] : pxp);
             const float ya = fminf(l.uym ? q[2][u] : pym, l.uyp ? q[2][u] : pyp);
             const float za = fminf(vprev, vnext);
+            const float fval = quad_bcast(q[3][u >> 2], u);
             float nv = vcur;
             bool c = false;
             if (p >= plo && p < phi && p != psrc) {
                 float qv;
-                if (UNI) qv = eik3d_f32_uniform(xa, ya, za, q[3][u]);
-                else qv = eik3d_f32_general(xa, ya, za, q[3][u], a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
+                if (UNI) qv = eik3d_f32_uniform(xa, ya, za, fval);
+                else qv = eik3d_f32_general(xa, ya, za, fval, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
                 c = qv < vcur;
                 nv = c ? qv : vcur;
             }
@@ -730,14 +784,27 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
         any_change = 0;
     }
 
+    // halo loads only in the lanes that use them (vector-memory instructions bound
+    // the sweep; a wave without a halo lane skips them)
+    const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
     auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
     auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
         const unsigned dm = (unsigned)(pb - l.pipj) * st;
 #pragma unroll
-        for (int u = 0; u < L; u++) {
-            ow[u] = ld2(l.oI + dm + (unsigned)(u + 2) * st);
-            hx[u] = ld2(l.oHX + dm + (unsigned)u * st);
-            hy[u] = ld2(l.oHY + dm + (unsigned)u * st);
+        for (int u = 0; u < L; u++) ow[u] = ld2(l.oI + dm + (unsigned)(u + 2) * st);
+        if (hxuse) {
+#pragma unroll
+            for (int u = 0; u < L; u++) hx[u] = ld2(l.oHX + dm + (unsigned)u * st);
+        } else {
+#pragma unroll
+            for (int u = 0; u < L; u++) hx[u] = make_float2(0.0f, 0.0f);
+        }
+        if (hyuse) {
+#pragma unroll
+            for (int u = 0; u < L; u++) hy[u] = ld2(l.oHY + dm + (unsigned)u * st);
+        } else {
+#pragma unroll
+            for (int u = 0; u < L; u++) hy[u] = make_float2(0.0f, 0.0f);
         }
     };
     const int m0 = -l.pipj;
@@ -1605,6 +1672,25 @@ __global__ void cell_time(const double *__restrict__ vp, float *__restrict__ f, 
     f[n] = uniform ? (float)(h / vp[n]) : (float)(1.0 / vp[n]);
 }
 
+// 8x8-tile-major copy of a cell-time array: element (k, ti, tj, li, lj) of ft is
+// f[k][8ti+li][8tj+lj], 0 where an edge tile runs past the grid.  One thread per
+// element of ft.
+__global__ void tile8_copy(const float *__restrict__ f, float *__restrict__ ft, int nrows,
+                           int ncols, long total)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int nti = (nrows + 7) / 8, ntj = (ncols + 7) / 8;
+    const int lj = (int)(e & 7), li = (int)((e >> 3) & 7);
+    const long t = e >> 6;
+    const int tj = (int)(t % ntj);
+    const long r = t / ntj;
+    const int ti = (int)(r % nti);
+    const long k = r / nti;
+    const int i = 8 * ti + li, j = 8 * tj + lj;
+    ft[e] = i < nrows && j < ncols ? f[(k * nrows + i) * ncols + j] : 0.0f;
+}
+
 }  // namespace
 
 namespace pfast {
@@ -1792,6 +1878,20 @@ hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, in
 {
     hipLaunchKernelGGL(gather_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, s, N,
                        PFAST_CHUNK, strip_sign, out);
+    return hipGetLastError();
+}
+
+long tile8_size(int nrows, int ncols, int nz)
+{
+    return (long)nz * ((nrows + 7) / 8) * ((ncols + 7) / 8) * 64;
+}
+
+hipError_t launch_tile8_copy(hipStream_t st, const float *f, float *ft, int nrows, int ncols,
+                             int nz)
+{
+    const long total = tile8_size(nrows, ncols, nz);
+    hipLaunchKernelGGL(tile8_copy, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f, ft,
+                       nrows, ncols, total);
     return hipGetLastError();
 }
 

@@ -845,7 +845,8 @@ int pfast_model3d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     m->uniform = dx == dy && dy == dz;
     const long N = (long)nrows * ncols * nz;
     int rc;
-    if ((rc = pfast::dalloc(ctx, &m->vp, N)) || (rc = pfast::dalloc(ctx, &m->f, N))) {
+    if ((rc = pfast::dalloc(ctx, &m->vp, N)) || (rc = pfast::dalloc(ctx, &m->f, N)) ||
+        (rc = pfast::dalloc(ctx, &m->ft, pfast::tile8_size(nrows, ncols, nz)))) {
         pfast_model3d_destroy(m);
         return rc;
     }
@@ -865,6 +866,7 @@ int pfast_model3d_set_vp(pfast_model3d *m, const double *vp)
     hipStream_t st = m->ctx->stream;
     HIPCHK(hipMemcpyAsync(m->vp, vp, N * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(pfast::launch_cell_time(st, m->vp, m->f, N, m->dx, m->uniform));
+    HIPCHK(pfast::launch_tile8_copy(st, m->f, m->ft, m->nrows, m->ncols, m->nz));
     HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;
 }
@@ -877,6 +879,7 @@ int pfast_model3d_destroy(pfast_model3d *m)
     m->ctx->nmodels--;
     pfast::dfree(m->ctx, m->vp);
     pfast::dfree(m->ctx, m->f);
+    pfast::dfree(m->ctx, m->ft);
     delete m;
     return PFAST_OK;
 }
@@ -1025,6 +1028,7 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         c.field[0] = s->T + (size_t)i * N * PFAST_CHUNK;
         c.field[1] = s->L ? s->L + (size_t)i * N * PFAST_CHUNK : nullptr;
         c.slow = m->f;
+        c.slow8 = m->ft;
         for (int k = 0; k < PFAST_CHUNK; k++) c.srcnode[k] = sn[i * PFAST_CHUNK + k];
     }
     if ((rc = pfast::dupload(m->ctx, &s->chunks, s->hchunks.data(), s->hchunks.size()))) {

@@ -15,6 +15,7 @@ struct pfast_model3d {
     bool uniform;
     double *vp = nullptr;  // N doubles
     float *f = nullptr;    // N cell times / slowness
+    float *ft = nullptr;   // the same, 8x8-tile-major (ChunkRef::slow8)
     int nshots = 0;        // live shot sets bound to this model
 };
 
