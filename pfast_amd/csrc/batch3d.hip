@@ -693,13 +693,17 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     // of the following batch into nq first
     auto batch = [&](int pb, float (*q)[L], float (*nq)[L]) {
         wait_vm_all();
+        // an opaque copy of the batch start for the addresses: otherwise the compiler
+        // keeps every per-step address of the queues in a register of its own
+        int pbo = pb;
+        asm volatile("" : "+s"(pbo));
         {  // changed nodes of the previous batch (steps pb-L .. pb-1)
-            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+            const unsigned d = (unsigned)(pbo - L - l.pipj) * st;
 #pragma unroll
             for (int u = 0; u < L; u++)
                 if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pb + L, nq[0], nq[1], nq[2], nq[3]);
+        load_batch(pbo + L, nq[0], nq[1], nq[2], nq[3]);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
@@ -825,13 +829,17 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
     }
     auto batch = [&](int pb, float2 (*q)[L], float2 (*nq)[L]) {
         wait_vm_all();
+        // an opaque copy of the batch start for the addresses: otherwise the compiler
+        // keeps every per-step address of the queues in a register of its own
+        int pbo = pb;
+        asm volatile("" : "+s"(pbo));
         {
-            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+            const unsigned d = (unsigned)(pbo - L - l.pipj) * st;
 #pragma unroll
             for (int u = 0; u < L; u++)
                 if (chg[u]) bstore(RL, l.oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pb + L, nq[0], nq[1], nq[2]);
+        load_batch(pbo + L, nq[0], nq[1], nq[2]);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;

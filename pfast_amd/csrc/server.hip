@@ -735,7 +735,8 @@ int pfast_server_add_model3d(pfast_server *S, const double *vp, int nrows, int n
     ARGCHK(S && vp && model_id, "NULL argument");
     HIPCHK(hipSetDevice(S->device));
     const size_t N = (size_t)nrows * ncols * nz;
-    const size_t bytes = round2m(N * 8) + round2m(N * 4);
+    const size_t bytes = round2m(N * 8) + round2m(N * 4) +
+                         round2m((size_t)pfast::tile8_size(nrows, ncols, nz) * 4);
     ARGCHK(S->in_use + bytes <= S->budget, "resident models exceed the server's memory budget");
     SModel M;
     M.dim = 3;
