@@ -1706,7 +1706,7 @@ namespace pfast {
 // Sweep kernel variants (schedule, tile rows x columns, batch/prefetch depth).
 // Variant 0 is the default; the others stay selectable for measurement.
 static const SweepVariant kVariants[] = {
-    {0, 8, 8, 6, "column-window 8x8 L6"},
+    {0, 8, 16, 6, "column-window 8x16 L6"},
     {1, 8, 8, 2, "slab-ring 8x8 L2"},
     {0, 8, 8, 4, "column-window 8x8 L4"},
     {0, 8, 8, 2, "column-window 8x8 L2"},
@@ -1716,13 +1716,16 @@ static const SweepVariant kVariants[] = {
     {0, 4, 4, 4, "column-window 4x4 L4"},
     {0, 4, 8, 6, "column-window 4x8 L6"},
     {0, 8, 4, 6, "column-window 8x4 L6"},
-    {0, 8, 16, 6, "column-window 8x16 L6"},
+    {0, 8, 8, 6, "column-window 8x8 L6"},
     {0, 16, 8, 6, "column-window 16x8 L6"},
     {0, 8, 16, 4, "column-window 8x16 L4"},
     {0, 8, 8, 8, "column-window 8x8 L8"},
     {3, 8, 8, 2, "vector-lane 8x8 L2"},
     {3, 8, 8, 4, "vector-lane 8x8 L4"},
     {3, 8, 16, 2, "vector-lane 8x16 L2"},
+    {0, 8, 32, 6, "column-window 8x32 L6"},
+    {0, 16, 16, 6, "column-window 16x16 L6"},
+    {0, 4, 16, 6, "column-window 4x16 L6"},
 };
 
 int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
@@ -1791,7 +1794,7 @@ hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int ad
     constexpr int C = PFAST_CHUNK;
     const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));
     switch (variant) {
-    case 0: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 0: launch_col<8, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 1: {
         constexpr int TI = 8, TJ = 8, L = 2;
         const dim3 block(TI * TJ * C);
@@ -1813,13 +1816,16 @@ hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int ad
     case 7: launch_col<4, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 8: launch_col<4, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 9: launch_col<8, 4, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 10: launch_col<8, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 10: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 11: launch_col<16, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 12: launch_col<8, 16, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 13: launch_col<8, 8, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 14: launch_vec<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 15: launch_vec<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     case 16: launch_vec<8, 16, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 17: launch_col<8, 32, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 18: launch_col<16, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
+    case 19: launch_col<4, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
