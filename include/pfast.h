@@ -155,7 +155,7 @@ int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable);
 int pfast_ctx_timing_reset(pfast_ctx *ctx);
 /* Batched sweep kernel variant of this context (0 = default).  Every variant
  * computes the same sweeps in the same per-node order (same results); they differ
- * in schedule and tile shape.  pfast_sweep_name: a variant's description, NULL
+ * in tile shape and batch depth.  pfast_sweep_name: a variant's description, NULL
  * past the last one. */
 int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant);
 const char *pfast_sweep_name(int variant);

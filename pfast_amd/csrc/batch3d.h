@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 
 // Sources per chunk (innermost dimension of the batched field layout).
-#define PFAST_CHUNK 4
+#define PFAST_CHUNK 8
 
 // One chunk of C sources as the sweep kernels see it.  A launch covers chunks
 // through a table of these, so the chunks of one launch may belong to different
@@ -36,10 +36,8 @@ struct SlabArgs {
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
 };
 
-// A sweep kernel variant: schedule (0 column window, 1 slab ring, 2 loader wave,
-// 3 vector lanes: a chunk's sources in one lane),
-// tile rows x columns, batch (column window), prefetch (slab ring) or forward
-// lookahead (loader wave) depth.
+// A sweep kernel variant: schedule (0: column window, the only one), tile rows x
+// columns, batch depth.
 struct SweepVariant {
     int kind, ti, tj, depth;
     const char *name;

@@ -6,35 +6,19 @@
 // time f[k][i][j] (h/v for uniform spacing, else 1/v) through a ChunkRef, so one
 // launch can sweep chunks of several shot sets and models of one grid shape.
 //
-// Schedule ("slab ring"): the (i,j) plane is cut into TI x TJ column tiles that
-// run the whole depth.  For one ordering (s3,s1,s2) tile (ti',tj') (coordinates
-// taken in sweep direction) depends only on tiles (ti'-1,tj') and (ti',tj'-1), so
-// one launch per tile anti-diagonal D = ti'+tj' keeps the reference's upwind
-// order with no inter-block waiting.  Inside a block, thread (li,lj,c) owns one
-// column of one source; hyperplane p of the tile processes level pk = p - pi - pj
-// (pi,pj its sweep-ordered position), so every node sees its behind neighbours
-// updated and its ahead neighbours not yet, as in the reference's lexicographic
-// sweep.  Depth slabs (the tile plus its four side halos) stream through an LDS
-// ring of W levels: each hyperplane the block stores the slab it will need next
-// into LDS (loaded L hyperplanes earlier into registers, so HBM latency overlaps
-// the sweep), updates its nodes in place, and writes back the slab that no
-// thread reads any more (only nodes that changed).
+// Schedule: the (i,j) plane is cut into TI x TJ column tiles that run the whole
+// depth.  For one ordering (s3,s1,s2) tile (ti',tj') (coordinates taken in sweep
+// direction) depends only on tiles (ti'-1,tj') and (ti',tj'-1), so one launch per
+// tile anti-diagonal D = ti'+tj' keeps the reference's upwind order with no
+// inter-block waiting.  Inside a block, thread (li,lj,c) owns one column of one
+// source; hyperplane p of the tile processes level pk = p - pi - pj (pi,pj its
+// sweep-ordered position), so every node sees its behind neighbours updated and
+// its ahead neighbours not yet, as in the reference's lexicographic sweep.
 #include "batch3d.h"
 #include "pfast_internal.h"
 #include "solvers.h"
 
 namespace {
-
-template <int TI, int TJ, int C>
-struct Geo {
-    static constexpr int NT = TI * TJ * C;
-    static constexpr int HI = TI + 2, HJ = TJ + 2;
-    static constexpr int W = TI + TJ + 3;  // ring depth >= span + 5
-    static constexpr int SLAB = HI * HJ * C;
-    static constexpr int NH = 2 * (TI + TJ) * C;  // side-halo elements per level
-    static_assert(NH <= NT, "halo must fit one element per thread");
-    static_assert(64 % C == 0, "C must divide the wavefront");
-};
 
 struct TileCtx {
     int chunk, i0, j0, TIe, TJe, span;
@@ -61,53 +45,6 @@ __device__ __forceinline__ TileCtx decode_tile(const SlabArgs &a, int D, int s1,
     return t;
 }
 
-// Side-halo element h of a level: LDS position and global (i,j), or !use.
-template <int TI, int TJ, int C>
-struct HaloSlot {
-    int lds;      // offset inside one ring slot
-    long gnode;   // i*ncols + j of the halo node (valid if inside)
-    int c;
-    bool use;     // this thread owns a halo element that is read
-    bool inside;  // halo node is inside the domain (else it reads INF)
-};
-
-template <int TI, int TJ, int C>
-__device__ __forceinline__ HaloSlot<TI, TJ, C> halo_slot(int h, const TileCtx &t, int nrows,
-                                                         int ncols)
-{
-    using G = Geo<TI, TJ, C>;
-    HaloSlot<TI, TJ, C> r;
-    r.use = false;
-    r.inside = false;
-    r.lds = 0;
-    r.gnode = 0;
-    r.c = h % C;
-    if (h >= G::NH) return r;
-    int q = h / C;  // node index along the side list
-    int hi, hj, gi, gj;
-    if (q < TJ) {  // top side, row i0-1
-        if (q >= t.TJe) return r;
-        hi = 0; hj = q + 1; gi = t.i0 - 1; gj = t.j0 + q;
-    } else if (q < 2 * TJ) {  // bottom side, row i0+TIe
-        q -= TJ;
-        if (q >= t.TJe) return r;
-        hi = t.TIe + 1; hj = q + 1; gi = t.i0 + t.TIe; gj = t.j0 + q;
-    } else if (q < 2 * TJ + TI) {  // left side, column j0-1
-        q -= 2 * TJ;
-        if (q >= t.TIe) return r;
-        hi = q + 1; hj = 0; gi = t.i0 + q; gj = t.j0 - 1;
-    } else {  // right side, column j0+TJe
-        q -= 2 * TJ + TI;
-        if (q >= t.TIe) return r;
-        hi = q + 1; hj = t.TJe + 1; gi = t.i0 + q; gj = t.j0 + t.TJe;
-    }
-    r.use = true;
-    r.inside = gi >= 0 && gi < nrows && gj >= 0 && gj < ncols;
-    r.lds = (hi * G::HJ + hj) * C + r.c;
-    r.gnode = (long)gi * ncols + gj;
-    return r;
-}
-
 __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
 {
     // bit c set iff some lane with (lane % C) == c voted
@@ -120,296 +57,8 @@ __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
     return bits;
 }
 
-// Per-thread constants of a tile task shared by the forward and adjoint sweeps.
-// Offsets are 32-bit element offsets inside one chunk's field (the chunk base
-// pointer is uniform), advanced by one level per hyperplane.
-template <int TI, int TJ, int C>
-struct Lane {
-    int c, li, lj, pipj, me;
-    bool valid, hreal, huse;
-    int hl;                 // halo LDS position (if huse)
-    unsigned oI, oH;        // element offsets of interior / halo node at level 0
-    int stepC;              // signed element step per level
-    unsigned node;          // i*ncols+j of this column (tile origin when !valid)
-};
-
-template <int TI, int TJ, int C>
-__device__ __forceinline__ Lane<TI, TJ, C> make_lane(const SlabArgs &a, const TileCtx &t, int s1,
-                                                     int s2, int s3)
-{
-    using G = Geo<TI, TJ, C>;
-    Lane<TI, TJ, C> l;
-    const int tid = threadIdx.x;
-    l.c = tid % C;
-    l.lj = (tid / C) % TJ;
-    l.li = tid / (C * TJ);
-    l.valid = l.li < t.TIe && l.lj < t.TJe;
-    l.pipj = (s1 > 0 ? l.li : t.TIe - 1 - l.li) + (s2 > 0 ? l.lj : t.TJe - 1 - l.lj);
-    l.me = ((l.li + 1) * G::HJ + (l.lj + 1)) * C + l.c;
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const unsigned pC = plane * C;
-    const unsigned org = (unsigned)t.i0 * a.ncols + t.j0;
-    l.node = l.valid ? (unsigned)(t.i0 + l.li) * a.ncols + (t.j0 + l.lj) : org;
-    const HaloSlot<TI, TJ, C> hs = halo_slot<TI, TJ, C>(tid, t, a.nrows, a.ncols);
-    l.huse = hs.use;
-    l.hreal = hs.use && hs.inside;
-    l.hl = hs.lds;
-    const unsigned hnode = l.hreal ? (unsigned)hs.gnode : org;
-    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
-    l.oI = k0 * pC + l.node * C + l.c;
-    l.oH = k0 * pC + hnode * C + hs.c;
-    l.stepC = s3 > 0 ? (int)pC : -(int)pC;
-    return l;
-}
-
-__device__ __forceinline__ int wrap_inc(int s, int W) { return s == W - 1 ? 0 : s + 1; }
-
-// ------------------------------------------------------------------- forward
-template <int TI, int TJ, int C, int L, bool UNI>
-__global__ __launch_bounds__(TI *TJ *C) void fwd3d_slab(SlabArgs a, int D, int s1, int s2,
-                                                        int s3)
-{
-    using G = Geo<TI, TJ, C>;
-    constexpr int W = G::W, SLAB = G::SLAB, FS = TI * TJ, SLOT = SLAB + FS, HJC = G::HJ * C;
-    __shared__ float ring[W * SLOT];  // per level: halo-padded slab, then the cell times
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const Lane<TI, TJ, C> l = make_lane<TI, TJ, C>(a, t, s1, s2, s3);
-    const int tid = threadIdx.x;
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    float *__restrict__ Fc = a.chunks[t.chunk].field[0];
-    const float *__restrict__ slow = a.chunks[t.chunk].slow;
-    // cell times: threads 0..FS-1 stream one node each
-    const bool fown = tid < FS && (tid / TJ) < t.TIe && (tid % TJ) < t.TJe;
-    const unsigned org = (unsigned)t.i0 * a.ncols + t.j0;
-    const unsigned fnode = fown ? (unsigned)(t.i0 + tid / TJ) * a.ncols + (t.j0 + tid % TJ) : org;
-    const int stepF = s3 > 0 ? (int)plane : -(int)plane;
-    unsigned oF = (s3 > 0 ? 0u : (unsigned)(nz - 1)) * plane + fnode;
-    unsigned oI = l.oI, oH = l.oH, oS = l.oI;
-    const int fme = SLAB + l.li * TJ + l.lj;
-    // level (in sweep order) at which this thread's column holds its own source
-    int srclev = -1;
-    {
-        const int sn = a.chunks[t.chunk].srcnode[l.c];
-        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
-            const int sk = (int)((unsigned)sn / plane);
-            srclev = s3 > 0 ? sk : nz - 1 - sk;
-        }
-    }
-    auto put = [&](int slot, float vi, float vh, float vf) {
-        float *r = ring + slot * SLOT;
-        if (l.valid) r[l.me] = vi;
-        if (l.huse) r[l.hl] = l.hreal ? vh : PFAST_INF_F;
-        if (fown) r[SLAB + tid] = vf;
-    };
-
-    // prologue: INF sentinel for level -1 (slot W-1), level 0 into slot 0, levels
-    // 1..L into registers
-    if (l.valid) ring[(W - 1) * SLOT + l.me] = PFAST_INF_F;
-    put(0, Fc[oI], Fc[oH], slow[oF]);
-    float rI[L], rH[L], rF[L];
-#pragma unroll
-    for (int u = 0; u < L; u++) {
-        rI[u] = rH[u] = rF[u] = 0.0f;
-        if (u + 1 < nz) {
-            oI += l.stepC;
-            oH += l.stepC;
-            oF += stepF;
-            rI[u] = Fc[oI];
-            rH[u] = Fc[oH];
-            rF[u] = slow[oF];
-        }
-    }
-
-    bool changed = false;
-    unsigned dirty = 0;  // bit (level % 32): this thread changed that level
-    const int pend = nz + t.span + 2;
-    int vs = l.pipj == 0 ? 0 : W - l.pipj;  // ring slot of this thread's level p - pipj
-    int ss = 1;                              // slot of level p+1
-    int sr = W - (t.span + 2);               // slot of the retired level p-span-2
-    for (int pb = 0; pb < pend; pb += L) {
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const int p = pb + u;
-            // (a) slab p+1 into the ring (INF sentinel past the last level);
-            // (b) prefetch slab p+1+L into registers
-            if (p + 1 < nz) put(ss, rI[u], rH[u], rF[u]);
-            else if (p + 1 == nz && l.valid) ring[ss * SLOT + l.me] = PFAST_INF_F;
-            if (p + 1 + L < nz) {
-                oI += l.stepC;
-                oH += l.stepC;
-                oF += stepF;
-                rI[u] = Fc[oI];
-                rH[u] = Fc[oH];
-                rF[u] = slow[oF];
-            }
-            lds_barrier();
-            // (d) update this thread's node at level pk
-            const int pk = p - l.pipj;
-            if (l.valid && pk >= 0 && pk < nz) {
-                float *r0 = ring + vs * SLOT;
-                const float self = r0[l.me];
-                const float xa = fminf(r0[l.me - C], r0[l.me + C]);
-                const float ya = fminf(r0[l.me - HJC], r0[l.me + HJC]);
-                const int vm = vs == 0 ? W - 1 : vs - 1, vp = wrap_inc(vs, W);
-                const float za = fminf(ring[vm * SLOT + l.me], ring[vp * SLOT + l.me]);
-                const float fv = r0[fme];
-                float q;
-                if (UNI) q = eik3d_f32_uniform(xa, ya, za, fv);
-                else q = eik3d_f32_general(xa, ya, za, fv, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
-                if (q < self && pk != srclev) {
-                    r0[l.me] = q;
-                    changed = true;
-                    dirty |= 1u << (pk & 31);
-                }
-            }
-            vs = wrap_inc(vs, W);
-            // (e) retire level p-span-2 (no thread reads it any more): write back
-            // the nodes this thread changed
-            const int lr = p - t.span - 2;
-            if (lr >= 0 && lr < nz) {
-                if (l.valid && ((dirty >> (lr & 31)) & 1u)) {
-                    Fc[oS] = ring[sr * SLOT + l.me];
-                    dirty &= ~(1u << (lr & 31));
-                }
-                oS += l.stepC;
-            }
-            sr = wrap_inc(sr, W);
-            ss = wrap_inc(ss, W);
-        }
-    }
-    const unsigned long long m = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && m) atomicOr(&a.changed[t.chunk], chunk_bits(m, C));
-}
-
-// ------------------------------------------------------------------- adjoint
-// lambda in field[1] of each chunk, traveltimes in field[0] (read-only; a negative value marks a
-// receiver node of that source, whose lambda is fixed by the boundary condition).
-// Only interior nodes are updated, so no edge sentinels are needed.
-template <int TI, int TJ, int C, int L>
-__global__ __launch_bounds__(TI *TJ *C) void adj3d_slab(SlabArgs a, int D, int s1, int s2,
-                                                        int s3)
-{
-    using G = Geo<TI, TJ, C>;
-    constexpr int W = G::W, SLAB = G::SLAB, HJC = G::HJ * C;
-    __shared__ float lring[W * SLAB];
-    __shared__ float tring[W * SLAB];
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const Lane<TI, TJ, C> l = make_lane<TI, TJ, C>(a, t, s1, s2, s3);
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    float *__restrict__ Lm = a.chunks[t.chunk].field[1];
-    const float *__restrict__ Tm = a.chunks[t.chunk].field[0];
-    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
-    const bool col_interior =
-        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
-    unsigned oI = l.oI, oH = l.oH, oS = l.oI;
-    auto put = [&](int slot, float vl, float vlh, float vt, float vth) {
-        const int o = slot * SLAB;
-        if (l.valid) {
-            lring[o + l.me] = vl;
-            tring[o + l.me] = vt;
-        }
-        if (l.huse) {
-            lring[o + l.hl] = vlh;
-            tring[o + l.hl] = l.hreal ? vth : PFAST_INF_F;
-        }
-    };
-    put(0, Lm[oI], Lm[oH], Tm[oI], Tm[oH]);
-    float rL[L], rLH[L], rT[L], rTH[L];
-#pragma unroll
-    for (int u = 0; u < L; u++) {
-        rL[u] = rLH[u] = rT[u] = rTH[u] = 0.0f;
-        if (u + 1 < nz) {
-            oI += l.stepC;
-            oH += l.stepC;
-            rL[u] = Lm[oI];
-            rLH[u] = Lm[oH];
-            rT[u] = Tm[oI];
-            rTH[u] = Tm[oH];
-        }
-    }
-    const float wx = a.wx, wy = a.wy, wz = a.wz;
-    bool changed = false;
-    unsigned dirty = 0;
-    const int pend = nz + t.span + 2;
-    int vs = l.pipj == 0 ? 0 : W - l.pipj;
-    int ss = 1;
-    int sr = W - (t.span + 2);
-    for (int pb = 0; pb < pend; pb += L) {
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const int p = pb + u;
-            if (p + 1 < nz) put(ss, rL[u], rLH[u], rT[u], rTH[u]);
-            if (p + 1 + L < nz) {
-                oI += l.stepC;
-                oH += l.stepC;
-                rL[u] = Lm[oI];
-                rLH[u] = Lm[oH];
-                rT[u] = Tm[oI];
-                rTH[u] = Tm[oH];
-            }
-            lds_barrier();
-            const int pk = p - l.pipj;
-            if (col_interior && pk >= 1 && pk <= nz - 2) {
-                const int o = vs * SLAB + l.me;
-                const float tc = tring[o];
-                if (!signbit(tc)) {
-                    // ring levels pk-1 / pk+1 are depths k-s3 / k+s3: name them by
-                    // absolute depth so that every ordering sums the terms in the
-                    // same order (another order rounds differently and makes a node
-                    // flip by an ulp from one ordering to the next, so the sweeps
-                    // would never become stationary)
-                    const int lo_ = (vs == 0 ? W - 1 : vs - 1) * SLAB + l.me;
-                    const int hi_ = wrap_inc(vs, W) * SLAB + l.me;
-                    const int om = s3 > 0 ? lo_ : hi_;  // depth k-1
-                    const int op = s3 > 0 ? hi_ : lo_;  // depth k+1
-                    const float txm = fabsf(tring[o - C]), txp = fabsf(tring[o + C]);
-                    const float tym = fabsf(tring[o - HJC]), typ = fabsf(tring[o + HJC]);
-                    const float tzm = fabsf(tring[om]), tzp = fabsf(tring[op]);
-                    float nw = 0.0f;
-                    const float I = PFAST_INF_F;
-                    if (!(tc >= I || txm >= I || txp >= I || tym >= I || typ >= I || tzm >= I ||
-                          tzp >= I)) {
-                        float den = 0.0f;
-                        float num = adj_term(tc, txp, lring[o + C], wx, den);
-                        num += adj_term(tc, txm, lring[o - C], wx, den);
-                        num += adj_term(tc, typ, lring[o + HJC], wy, den);
-                        num += adj_term(tc, tym, lring[o - HJC], wy, den);
-                        num += adj_term(tc, tzp, lring[op], wz, den);
-                        num += adj_term(tc, tzm, lring[om], wz, den);
-                        nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
-                    }
-                    if (nw != lring[o]) {
-                        lring[o] = nw;
-                        changed = true;
-                        dirty |= 1u << (pk & 31);
-                    }
-                }
-            }
-            vs = wrap_inc(vs, W);
-            const int lr = p - t.span - 2;
-            if (lr >= 0 && lr < nz) {
-                if (l.valid && ((dirty >> (lr & 31)) & 1u)) {
-                    Lm[oS] = lring[sr * SLAB + l.me];
-                    dirty &= ~(1u << (lr & 31));
-                }
-                oS += l.stepC;
-            }
-            sr = wrap_inc(sr, W);
-            ss = wrap_inc(ss, W);
-        }
-    }
-    const unsigned long long m = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && m) atomicOr(&a.changed[t.chunk], chunk_bits(m, C));
-}
-
 // ============================================================ column sweeps
-// Second schedule for the same sweep ("column window"): thread (li,lj,c) keeps its
+// The column window: thread (li,lj,c) keeps its
 // own column of one source in registers -- levels m-1 (updated), m, m+1 (not yet
 // updated) of the level m = p - pi - pj it works on at step p -- so depth needs no
 // exchange at all.  Only the in-plane neighbours do: at every step each thread
@@ -590,7 +239,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
-    static_assert(C == 4, "a node's sources fill one DPP quad");
+    static_assert(C % 4 == 0, "a node's sources fill whole DPP quads");
     // per step parity: [m-1 | m+1][thread], then an INF slot (domain-edge neighbour)
     constexpr int PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
@@ -641,10 +290,11 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
     // costs the address unit the same whatever its lanes hold), so: halo loads run
     // only in the lanes that use them (a wave without such a lane skips them), and
     // the cell time, one value per node, is loaded for four levels at once, lane c
-    // of the node's four lanes taking level 4g + c; step u broadcasts it by DPP.
+    // of each DPP quad of the node's lanes taking level 4g + c%4; step u broadcasts
+    // it within the quad.
     constexpr int FQ = (L + 3) / 4;
     const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
-    const unsigned oFc = oF + (unsigned)l.c * stF;
+    const unsigned oFc = oF + (unsigned)(l.c & 3) * stF;
     // queue of one batch: own column two levels ahead (m+2, INF outside the column),
     // halos at level m itself, cell time of levels m .. m+4*FQ-1
     auto load_batch = [&](int pb, float *ow, float *hx, float *hy, float *fv) {
@@ -902,640 +552,6 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1
                 &any_change);
 }
 
-// ============================================================ vector-lane sweeps
-// Fourth schedule ("vector lanes"): the column-window sweep with one lane per
-// column and the chunk's C = 4 sources of a node (its 16-byte record) in that lane.
-// Loads, the LDS exchange and stores move whole records; the four updates of a node
-// run as independent instruction chains, and what depends only on the node
-// (addresses, masks, neighbour selection, the cell time) is done once for the
-// four.  An 8x8 tile is one wave.  Same steps and the same per-node arithmetic as
-// fwd3d_col / adj3d_col, so the same results bit for bit; a record is stored when
-// any of its sources changed (the others rewrite their own value).
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ f32x4 bload4(rsrc_t r, unsigned off)
-{
-    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
-__device__ __forceinline__ void bstore4(rsrc_t r, unsigned off, f32x4 v)
-{
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
-}
-
-// Column lane of a vector tile: indices and LDS slots as a one-source column lane,
-// field offsets in 16-byte node records (cell-time offsets stay 4-byte).
-template <int TI, int TJ>
-__device__ __forceinline__ ColLane<TI, TJ, 1> make_vec_lane(const SlabArgs &a, const TileCtx &t,
-                                                            int s1, int s2, int s3)
-{
-    ColLane<TI, TJ, 1> l = make_col_lane<TI, TJ, 1>(a, t, s1, s2, s3);
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
-    const unsigned NONE = 0x80000000u;  // out of range at every level (never loaded)
-    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
-    const int hxj = l.uxm ? t.j0 - 1 : t.j0 + t.TJe;
-    const int hyi = l.uym ? t.i0 - 1 : t.i0 + t.TIe;
-    l.oI = 16u * (k0 * plane + l.node);
-    l.oHX = l.uxm || l.uxp ? 16u * (k0 * plane + (unsigned)gi * a.ncols + hxj) : NONE;
-    l.oHY = l.uym || l.uyp ? 16u * (k0 * plane + (unsigned)hyi * a.ncols + gj) : NONE;
-    l.step = s3 > 0 ? 16u * plane : 0u - 16u * plane;
-    return l;
-}
-
-// per-chunk changed bits of a wave: bit c iff some lane changed source c
-__device__ __forceinline__ void vote_changed(const SlabArgs &a, int chunk, unsigned changed)
-{
-    unsigned bits = 0;
-#pragma unroll
-    for (int c = 0; c < PFAST_CHUNK; c++)
-        if (__ballot((changed >> c) & 1u)) bits |= 1u << c;
-    if ((threadIdx.x & 63) == 0 && bits) atomicOr(&a.changed[chunk], bits);
-}
-
-template <int TI, int TJ, int L, bool UNI>
-__global__ __launch_bounds__(TI *TJ) void fwd3d_vec(SlabArgs a, int D, int s1, int s2, int s3)
-{
-    constexpr int NT = TI * TJ, C = PFAST_CHUNK;
-    static_assert(C == 4, "one 16-byte record per lane");
-    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
-    constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][lane], INF slot
-    __shared__ f32x4 pub[2 * PB];
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const ColLane<TI, TJ, 1> l = make_vec_lane<TI, TJ>(a, t, s1, s2, s3);
-    const int tid = threadIdx.x;
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], 16u * plane * (unsigned)nz);
-    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
-    const float INF = PFAST_INF_F;
-    const f32x4 INF4 = {INF, INF, INF, INF};
-    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF4;
-    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
-    int psrc[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) {
-        const int sn = a.chunks[t.chunk].srcnode[c];
-        psrc[c] = -1;
-        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
-            const int sk = (int)((unsigned)sn / plane);
-            psrc[c] = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
-        }
-    }
-    const unsigned st = l.step, stF = l.stepF;
-    auto load_batch = [&](int pb, f32x4 *ow, f32x4 *hx, f32x4 *hy, float *fv) {
-        const int mb = pb - l.pipj;
-        const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const f32x4 o = bload4(RF, l.oI + dm + (unsigned)(u + 2) * st);
-            ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF4;
-            hx[u] = bload4(RF, l.oHX + dm + (unsigned)u * st);
-            hy[u] = bload4(RF, l.oHY + dm + (unsigned)u * st);
-            fv[u] = bload(RS, l.oF + dmf + (unsigned)u * stF);
-        }
-    };
-    const int m0 = -l.pipj;
-    f32x4 vprev = INF4;
-    f32x4 vcur = l.valid && m0 == 0 ? bload4(RF, l.oI) : INF4;
-    f32x4 vnext = l.valid && m0 >= -1 ? bload4(RF, l.oI + (unsigned)(m0 + 1) * st) : INF4;
-    f32x4 qa[3][L], qb[3][L];
-    float fa[L], fb[L];
-    load_batch(0, qa[0], qa[1], qa[2], fa);
-    __syncthreads();  // INF slot visible
-
-    unsigned changed = 0;
-    f32x4 sval[L];
-    bool chg[L];
-#pragma unroll
-    for (int u = 0; u < L; u++) {
-        sval[u] = INF4;
-        chg[u] = false;
-    }
-    auto batch = [&](int pb, f32x4 (*q)[L], const float *qf, f32x4 (*nq)[L], float *nqf) {
-        wait_vm_all();
-        {  // records changed in the previous batch (steps pb-L .. pb-1)
-            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
-#pragma unroll
-            for (int u = 0; u < L; u++)
-                if (chg[u]) bstore4(RF, l.oI + d + (unsigned)u * st, sval[u]);
-        }
-        load_batch(pb + L, nq[0], nq[1], nq[2], nqf);
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const int p = pb + u;
-            f32x4 *P = pub + (u & 1) * PB;
-            P[tid] = vprev;
-            P[NT + tid] = vnext;
-            lds_barrier();
-            const f32x4 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            const f32x4 xm = l.uxm ? q[1][u] : pxm, xp = l.uxp ? q[1][u] : pxp;
-            const f32x4 ym = l.uym ? q[2][u] : pym, yp = l.uyp ? q[2][u] : pyp;
-            const bool live = p >= plo && p < phi;
-            f32x4 nv = vcur;
-            unsigned cm = 0;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const float xa = fminf(xm[c], xp[c]), ya = fminf(ym[c], yp[c]);
-                const float za = fminf(vprev[c], vnext[c]);
-                float qv;
-                if (UNI) qv = eik3d_f32_uniform(xa, ya, za, qf[u]);
-                else qv = eik3d_f32_general(xa, ya, za, qf[u], a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
-                const bool cc = live && p != psrc[c] && qv < vcur[c];
-                nv[c] = cc ? qv : vcur[c];
-                cm |= cc ? 1u << c : 0u;
-            }
-            chg[u] = cm != 0;
-            changed |= cm;
-            sval[u] = nv;
-            vprev = nv;
-            vcur = vnext;
-            vnext = q[0][u];
-        }
-    };
-    const int pend = nz + t.span;
-    for (int pb = 0; pb < pend; pb += 2 * L) {
-        batch(pb, qa, fa, qb, fb);
-        batch(pb + L, qb, fb, qa, fa);
-    }
-    {
-        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
-        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
-#pragma unroll
-        for (int u = 0; u < L; u++)
-            if (chg[u]) bstore4(RF, l.oI + d + (unsigned)u * st, sval[u]);
-    }
-    vote_changed(a, t.chunk, changed);
-}
-
-template <int TI, int TJ, int L, bool S3POS>
-__global__ __launch_bounds__(TI *TJ) void adj3d_vec(SlabArgs a, int D, int s1, int s2)
-{
-    constexpr int NT = TI * TJ, C = PFAST_CHUNK;
-    constexpr int s3 = S3POS ? 1 : -1;
-    static_assert(C == 4, "one 16-byte record per lane");
-    static_assert(L % 2 == 0, "the step parity must be static inside a batch");
-    constexpr int PB = 2 * NT + 1;
-    __shared__ f32x4 pubL[2 * PB];  // lambda records
-    __shared__ f32x4 pubT[2 * PB];  // traveltime records (negative: receiver node)
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const ColLane<TI, TJ, 1> l = make_vec_lane<TI, TJ>(a, t, s1, s2, s3);
-    const int tid = threadIdx.x;
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const unsigned fbytes = 16u * plane * (unsigned)nz;
-    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
-    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
-    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
-    const bool col_interior =
-        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
-    const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
-    const unsigned st = l.step;
-    const float wx = a.wx, wy = a.wy, wz = a.wz;
-    const float I = PFAST_INF_F;
-    const f32x4 Z4 = {0.0f, 0.0f, 0.0f, 0.0f}, INF4 = {I, I, I, I};
-    if (tid == 0) {
-        pubL[2 * NT] = pubL[PB + 2 * NT] = Z4;
-        pubT[2 * NT] = pubT[PB + 2 * NT] = INF4;
-    }
-    auto load_batch = [&](int pb, f32x4 (*q)[L]) {
-        const unsigned dm = (unsigned)(pb - l.pipj) * st;
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const unsigned o = l.oI + dm + (unsigned)(u + 2) * st;
-            const unsigned ox = l.oHX + dm + (unsigned)u * st, oy = l.oHY + dm + (unsigned)u * st;
-            q[0][u] = bload4(RL, o);
-            q[1][u] = bload4(RT, o);
-            q[2][u] = bload4(RL, ox);
-            q[3][u] = bload4(RT, ox);
-            q[4][u] = bload4(RL, oy);
-            q[5][u] = bload4(RT, oy);
-        }
-    };
-    const int m0 = -l.pipj;
-    f32x4 pL = Z4, pT = Z4;  // level m-1
-    f32x4 cL = l.valid && m0 == 0 ? bload4(RL, l.oI) : Z4;
-    f32x4 cT = l.valid && m0 == 0 ? bload4(RT, l.oI) : Z4;
-    const unsigned on = l.oI + (unsigned)(m0 + 1) * st;
-    f32x4 nL = l.valid && m0 >= -1 ? bload4(RL, on) : Z4;
-    f32x4 nT = l.valid && m0 >= -1 ? bload4(RT, on) : Z4;
-    f32x4 qa[6][L], qb[6][L];
-    load_batch(0, qa);
-    __syncthreads();
-
-    unsigned changed = 0;
-    f32x4 sval[L];
-    bool chg[L];
-#pragma unroll
-    for (int u = 0; u < L; u++) {
-        sval[u] = Z4;
-        chg[u] = false;
-    }
-    auto batch = [&](int pb, f32x4 (*q)[L], f32x4 (*nq)[L]) {
-        wait_vm_all();
-        {
-            const unsigned d = (unsigned)(pb - L - l.pipj) * st;
-#pragma unroll
-            for (int u = 0; u < L; u++)
-                if (chg[u]) bstore4(RL, l.oI + d + (unsigned)u * st, sval[u]);
-        }
-        load_batch(pb + L, nq);
-#pragma unroll
-        for (int u = 0; u < L; u++) {
-            const int p = pb + u;
-            f32x4 *PL = pubL + (u & 1) * PB, *PT = pubT + (u & 1) * PB;
-            PL[tid] = pL;
-            PT[tid] = pT;
-            PL[NT + tid] = nL;
-            PT[NT + tid] = nT;
-            lds_barrier();
-            const f32x4 xmL = l.uxm ? q[2][u] : PL[l.rxm], xmT = l.uxm ? q[3][u] : PT[l.rxm];
-            const f32x4 xpL = l.uxp ? q[2][u] : PL[l.rxp], xpT = l.uxp ? q[3][u] : PT[l.rxp];
-            const f32x4 ymL = l.uym ? q[4][u] : PL[l.rym], ymT = l.uym ? q[5][u] : PT[l.rym];
-            const f32x4 ypL = l.uyp ? q[4][u] : PL[l.ryp], ypT = l.uyp ? q[5][u] : PT[l.ryp];
-            // depth k-1 / k+1 by absolute depth, as in adj3d_col
-            const f32x4 zmL = S3POS ? pL : nL, zmT = S3POS ? pT : nT;
-            const f32x4 zpL = S3POS ? nL : pL, zpT = S3POS ? nT : pT;
-            const bool live = p >= plo && p < phi;
-            f32x4 nl = cL;
-            unsigned cm = 0;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const float tc = cT[c];
-                const float txm = fabsf(xmT[c]), txp = fabsf(xpT[c]), tym = fabsf(ymT[c]),
-                            typ = fabsf(ypT[c]), tzm = fabsf(zmT[c]), tzp = fabsf(zpT[c]);
-                const bool anyinf = tc >= I || txm >= I || txp >= I || tym >= I || typ >= I ||
-                                    tzm >= I || tzp >= I;
-                float den = 0.0f;
-                float num = adj_term(tc, txp, xpL[c], wx, den);
-                num += adj_term(tc, txm, xmL[c], wx, den);
-                num += adj_term(tc, typ, ypL[c], wy, den);
-                num += adj_term(tc, tym, ymL[c], wy, den);
-                num += adj_term(tc, tzp, zpL[c], wz, den);
-                num += adj_term(tc, tzm, zmL[c], wz, den);
-                const float nw = !anyinf && fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
-                const bool act = live && !signbit(tc);
-                const bool cc = act && nw != cL[c];
-                nl[c] = act ? nw : cL[c];
-                cm |= cc ? 1u << c : 0u;
-            }
-            chg[u] = cm != 0;
-            changed |= cm;
-            sval[u] = nl;
-            pL = nl;
-            pT = cT;
-            cL = nL;
-            cT = nT;
-            nL = q[0][u];
-            nT = q[1][u];
-        }
-    };
-    const int pend = nz + t.span;
-    for (int pb = 0; pb < pend; pb += 2 * L) {
-        batch(pb, qa, qb);
-        batch(pb + L, qb, qa);
-    }
-    {
-        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
-        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
-#pragma unroll
-        for (int u = 0; u < L; u++)
-            if (chg[u]) bstore4(RL, l.oI + d + (unsigned)u * st, sval[u]);
-    }
-    vote_changed(a, t.chunk, changed);
-}
-
-// ============================================================ loader-wave sweeps
-// Third schedule ("loader wave"): the column-window sweep with its HBM reads moved
-// out of the compute waves.  One extra wave per block only issues loads: at step p
-// it streams the data every compute thread needs at step p + A (own column two
-// levels ahead, tile-edge halos, cell time) from HBM straight into a step-indexed
-// LDS ring (buffer -> LDS DMA; one dwordx4 lane carries the C = 4 sources of a
-// column), then waits until step p + 1's data has landed and joins the step
-// barrier.  Because that wave issues loads only, vmcnt counts them in order and
-// the wait is exact, so A steps of HBM latency are hidden.  The compute waves read
-// everything from LDS and store changed nodes as they go (never waiting on them).
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-__device__ __forceinline__ void dma16(rsrc_t r, void *lds, unsigned off)
-{
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t *)lds, 16, off, 0, 0, 0);
-}
-__device__ __forceinline__ void dma4(rsrc_t r, void *lds, unsigned off)
-{
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t *)lds, 4, off, 0, 0, 0);
-}
-// s_waitcnt vmcnt(n), n <= 63 (expcnt / lgkmcnt at their maxima)
-template <int N>
-__device__ __forceinline__ void wait_vm()
-{
-    static_assert(N >= 0 && N <= 63, "vmcnt is 6 bits");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
-}
-
-// The consumer (compute thread) of a tile-edge halo entry h of a step slot, and
-// the halo node it needs.  Entries 0 .. 2TI-1: x halos (row h/2, side h%2: 0 = x-,
-// 1 = x+); then 2TJ y halos (column, side 0 = y-, 1 = y+).
-template <int TI, int TJ>
-__device__ __forceinline__ bool halo_entry(int h, const TileCtx &t, int nrows, int ncols, int &cli,
-                                           int &clj, unsigned &hnode)
-{
-    int gi, gj;
-    if (h < 2 * TI) {
-        cli = h / 2;
-        const int side = h % 2;
-        clj = side ? t.TJe - 1 : 0;
-        gi = t.i0 + cli;
-        gj = side ? t.j0 + t.TJe : t.j0 - 1;
-        if (cli >= t.TIe) return false;
-    } else {
-        clj = (h - 2 * TI) / 2;
-        const int side = (h - 2 * TI) % 2;
-        cli = side ? t.TIe - 1 : 0;
-        gi = side ? t.i0 + t.TIe : t.i0 - 1;
-        gj = t.j0 + clj;
-        if (clj >= t.TJe) return false;
-    }
-    if (gi < 0 || gi >= nrows || gj < 0 || gj >= ncols) return false;
-    hnode = (unsigned)gi * ncols + gj;
-    return true;
-}
-
-template <int TI, int TJ>
-__device__ __forceinline__ int pipj_of(int li, int lj, const TileCtx &t, int s1, int s2)
-{
-    return (s1 > 0 ? li : t.TIe - 1 - li) + (s2 > 0 ? lj : t.TJe - 1 - lj);
-}
-
-template <int TI, int TJ, int C, int A, bool UNI>
-__global__ __launch_bounds__(TI *TJ *C + 64) void fwd3d_dma(SlabArgs a, int D, int s1, int s2,
-                                                            int s3)
-{
-    static_assert(C == 4, "one dwordx4 lane per column");
-    constexpr int NT = TI * TJ * C, NCOL = TI * TJ, NH = 2 * (TI + TJ);
-    static_assert(NCOL <= 64 && NH <= 64, "one loader lane per column / halo node");
-    constexpr int R = A + 1;  // ring slots: step p + A reuses the slot of step p - 1
-    constexpr int S_HALO = NT, S_F = NT + NH * C, SLOT = S_F + NCOL;  // floats per slot
-    static_assert(SLOT % 4 == 0, "16-byte aligned slots");
-    constexpr int PB = 2 * NT + 1;
-    __shared__ float pub[2 * PB];
-    __shared__ __attribute__((aligned(16))) float ring[R * SLOT];
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const int tid = threadIdx.x;
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
-    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], 4u * pC * nz);
-    const rsrc_t RS = make_rsrc(a.chunks[t.chunk].slow, 4u * plane * nz);
-    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
-    const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
-    const unsigned stF = s3 > 0 ? 4u * plane : 0u - 4u * plane;
-    const unsigned NONE = 0x80000000u;
-    const float INF = PFAST_INF_F;
-    const int pend = nz + t.span;
-    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = INF;
-    __syncthreads();
-
-    if (tid >= NT) {  // ------------------------------------------ loader wave
-        const int l = tid - NT;
-        // own column: level p - pipj + 2 for step p
-        unsigned oOwn = NONE, oF = NONE, oH = NONE;
-        if (l < NCOL) {
-            const int li = l / TJ, lj = l % TJ;
-            if (li < t.TIe && lj < t.TJe) {
-                const int pp = pipj_of<TI, TJ>(li, lj, t, s1, s2);
-                const unsigned node = (unsigned)(t.i0 + li) * a.ncols + t.j0 + lj;
-                oOwn = 4u * (k0 * pC + node * C) + (unsigned)(2 - pp) * st;
-                oF = 4u * (k0 * plane + node) + (unsigned)(-pp) * stF;
-            }
-        }
-        if (l < NH) {
-            int cli, clj;
-            unsigned hn;
-            if (halo_entry<TI, TJ>(l, t, a.nrows, a.ncols, cli, clj, hn))
-                oH = 4u * (k0 * pC + hn * C) + (unsigned)(-pipj_of<TI, TJ>(cli, clj, t, s1, s2)) * st;
-        }
-        // the data of step q into slot q % R; each DMA only on the lanes that own an
-        // entry (a lane writes its 16 / 4 bytes at lane * size from the base even
-        // when its load is out of range), lane 0 always among them, so every issue
-        // is exactly three VMEM instructions
-        auto issue = [&](int q) {
-            float *S = ring + (q % R) * SLOT;
-            if (l < NCOL) {
-                dma16(RF, S, oOwn);
-                dma4(RS, S + S_F, oF);
-            }
-            if (l < NH) dma16(RF, S + S_HALO, oH);
-            oOwn += st;
-            oH += st;
-            oF += stF;
-        };
-        for (int q = 0; q < A; q++) issue(q);
-        for (int p = 0; p < pend; p++) {
-            wait_vm<3 * (A - 1)>();  // step p's three loads have landed
-            __builtin_amdgcn_s_barrier();
-            issue(p + A);  // slot of step p - 1, whose readers passed this barrier
-        }
-        wait_vm<0>();
-        return;
-    }
-
-    // ------------------------------------------------------------ compute waves
-    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
-    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
-    int psrc = -1;
-    {
-        const int sn = a.chunks[t.chunk].srcnode[l.c];
-        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
-            const int sk = (int)((unsigned)sn / plane);
-            psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
-        }
-    }
-    // ring indices of this thread's entries
-    const int iOwn = tid;
-    const int col = l.li * TJ + l.lj;
-    const int iF = S_F + col;
-    const int iHX = S_HALO + (l.li * 2 + (l.uxp ? 1 : 0)) * C + l.c;
-    const int iHY = S_HALO + (2 * TI + l.lj * 2 + (l.uyp ? 1 : 0)) * C + l.c;
-    const int m0 = -l.pipj;
-    float vprev = INF;
-    float vcur = l.valid && m0 == 0 ? bload(RF, l.oI) : INF;
-    float vnext = l.valid && m0 >= -1 ? bload(RF, l.oI + (unsigned)(m0 + 1) * l.step) : INF;
-    unsigned oS = l.oI + (unsigned)m0 * l.step;  // byte offset of this thread's level at step p
-    bool changed = false;
-    int slot = 0;
-    for (int p = 0; p < pend; p++) {
-        float *P = pub + (p & 1) * PB;
-        P[tid] = vprev;
-        P[NT + tid] = vnext;
-        lds_barrier();
-        const float *S = ring + slot * SLOT;
-        const float own = S[iOwn], hx = S[iHX], hy = S[iHY], fv = S[iF];
-        const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-        const float xa = fminf(l.uxm ? hx : pxm, l.uxp ? hx : pxp);
-        const float ya = fminf(l.uym ? hy : pym, l.uyp ? hy : pyp);
-        const float za = fminf(vprev, vnext);
-        float nv = vcur;
-        if (p >= plo && p < phi && p != psrc) {
-            float qv;
-            if (UNI) qv = eik3d_f32_uniform(xa, ya, za, fv);
-            else qv = eik3d_f32_general(xa, ya, za, fv, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
-            if (qv < vcur) {
-                nv = qv;
-                bstore(RF, oS, nv);
-                changed = true;
-            }
-        }
-        vprev = nv;
-        vcur = vnext;
-        vnext = (unsigned)(p + 2 - l.pipj) < (unsigned)nz ? own : INF;
-        oS += l.step;
-        slot = slot == R - 1 ? 0 : slot + 1;
-    }
-    const unsigned long long bm = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-}
-
-template <int TI, int TJ, int C, int A, bool S3POS>
-__global__ __launch_bounds__(TI *TJ *C + 64) void adj3d_dma(SlabArgs a, int D, int s1, int s2)
-{
-    static_assert(C == 4, "one dwordx4 lane per column");
-    constexpr int s3 = S3POS ? 1 : -1;
-    constexpr int NT = TI * TJ * C, NCOL = TI * TJ, NH = 2 * (TI + TJ);
-    static_assert(NCOL <= 64 && NH <= 64, "one loader lane per column / halo node");
-    constexpr int R = A + 1;
-    // per slot: own lambda [NT] | own T [NT] | halo lambda [NH*C] | halo T [NH*C]
-    constexpr int S_OT = NT, S_HL = 2 * NT, S_HT = 2 * NT + NH * C, SLOT = 2 * NT + 2 * NH * C;
-    constexpr int PB = 2 * NT + 1;
-    __shared__ float2 pub[2 * PB];
-    __shared__ __attribute__((aligned(16))) float ring[R * SLOT];
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const int tid = threadIdx.x;
-    const int nz = a.nz;
-    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
-    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], 4u * pC * nz);
-    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], 4u * pC * nz);
-    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(nz - 1);
-    const unsigned st = s3 > 0 ? 4u * pC : 0u - 4u * pC;
-    const unsigned NONE = 0x80000000u;
-    const int pend = nz + t.span;
-    if (tid == 0) pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
-    __syncthreads();
-
-    if (tid >= NT) {  // ------------------------------------------ loader wave
-        const int l = tid - NT;
-        unsigned oOwn = NONE, oH = NONE;
-        if (l < NCOL) {
-            const int li = l / TJ, lj = l % TJ;
-            if (li < t.TIe && lj < t.TJe) {
-                const int pp = pipj_of<TI, TJ>(li, lj, t, s1, s2);
-                const unsigned node = (unsigned)(t.i0 + li) * a.ncols + t.j0 + lj;
-                oOwn = 4u * (k0 * pC + node * C) + (unsigned)(2 - pp) * st;
-            }
-        }
-        if (l < NH) {
-            int cli, clj;
-            unsigned hn;
-            if (halo_entry<TI, TJ>(l, t, a.nrows, a.ncols, cli, clj, hn))
-                oH = 4u * (k0 * pC + hn * C) + (unsigned)(-pipj_of<TI, TJ>(cli, clj, t, s1, s2)) * st;
-        }
-        auto issue = [&](int q) {  // four VMEM instructions (lane 0 is in both groups)
-            float *S = ring + (q % R) * SLOT;
-            if (l < NCOL) {
-                dma16(RL, S, oOwn);
-                dma16(RT, S + S_OT, oOwn);
-            }
-            if (l < NH) {
-                dma16(RL, S + S_HL, oH);
-                dma16(RT, S + S_HT, oH);
-            }
-            oOwn += st;
-            oH += st;
-        };
-        for (int q = 0; q < A; q++) issue(q);
-        for (int p = 0; p < pend; p++) {
-            wait_vm<4 * (A - 1)>();
-            __builtin_amdgcn_s_barrier();
-            issue(p + A);
-        }
-        wait_vm<0>();
-        return;
-    }
-
-    const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
-    const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
-    const bool col_interior =
-        l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
-    const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
-    const float wx = a.wx, wy = a.wy, wz = a.wz;
-    const int iHX = (l.li * 2 + (l.uxp ? 1 : 0)) * C + l.c;
-    const int iHY = (2 * TI + l.lj * 2 + (l.uyp ? 1 : 0)) * C + l.c;
-    auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
-    const int m0 = -l.pipj;
-    float2 vprev = make_float2(0.0f, 0.0f);
-    float2 vcur = l.valid && m0 == 0 ? ld2(l.oI) : make_float2(0.0f, 0.0f);
-    float2 vnext = l.valid && m0 >= -1 ? ld2(l.oI + (unsigned)(m0 + 1) * l.step) : make_float2(0.0f, 0.0f);
-    unsigned oS = l.oI + (unsigned)m0 * l.step;
-    bool changed = false;
-    int slot = 0;
-    for (int p = 0; p < pend; p++) {
-        float2 *P = pub + (p & 1) * PB;
-        P[tid] = vprev;
-        P[NT + tid] = vnext;
-        lds_barrier();
-        const float *S = ring + slot * SLOT;
-        const float2 own = make_float2(S[tid], S[S_OT + tid]);
-        const float2 hx = make_float2(S[S_HL + iHX], S[S_HT + iHX]);
-        const float2 hy = make_float2(S[S_HL + iHY], S[S_HT + iHY]);
-        const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-        float nl = vcur.x;
-        if (p >= plo && p < phi && !signbit(vcur.y)) {
-            const float2 xm = l.uxm ? hx : pxm;
-            const float2 xp = l.uxp ? hx : pxp;
-            const float2 ym = l.uym ? hy : pym;
-            const float2 yp = l.uyp ? hy : pyp;
-            // depth k-1 / k+1 by absolute depth (see adj3d_col)
-            const float2 zm = S3POS ? vprev : vnext;
-            const float2 zp = S3POS ? vnext : vprev;
-            const float tc = vcur.y;
-            const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
-                        typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
-            float nw = 0.0f;
-            const float I = PFAST_INF_F;
-            if (!(tc >= I || txm >= I || txp >= I || tym >= I || typ >= I || tzm >= I || tzp >= I)) {
-                float den = 0.0f;
-                float num = adj_term(tc, txp, xp.x, wx, den);
-                num += adj_term(tc, txm, xm.x, wx, den);
-                num += adj_term(tc, typ, yp.x, wy, den);
-                num += adj_term(tc, tym, ym.x, wy, den);
-                num += adj_term(tc, tzp, zp.x, wz, den);
-                num += adj_term(tc, tzm, zm.x, wz, den);
-                nw = fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
-            }
-            if (nw != vcur.x) {
-                bstore(RL, oS, nw);
-                changed = true;
-            }
-            nl = nw;
-        }
-        vprev = make_float2(nl, vcur.y);
-        vcur = vnext;
-        vnext = own;  // out-of-column levels read 0 and are never used by interior nodes
-        oS += l.step;
-        slot = slot == R - 1 ? 0 : slot + 1;
-    }
-    const unsigned long long bm = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-}
-
 // ------------------------------------------------------------------- helpers
 // T = INF everywhere, 0 at each source's node (padding sources have node -1).
 template <int C>
@@ -1630,21 +646,37 @@ __global__ void adj_misfit(const float *__restrict__ T, const long *__restrict__
     if (threadIdx.x == 0) EE[s] = sum;
 }
 
-// grad[n] = -(sum_s lambda_s[n]) / vp[n]^3, Kahan-summed in FP64 over sources.
+// grad[n] = -(sum_s lambda_s[n]) / vp[n]^3, Kahan-summed in FP64 over sources in
+// source order.  A node's C values of a chunk are one 16-byte-aligned record, read
+// with 16-byte loads.
+template <int C>
 __global__ void grad_reduce(const float *__restrict__ Lm, const double *__restrict__ vp,
-                            double *__restrict__ g, long N, int nchunk, int C, int nsrc)
+                            double *__restrict__ g, long N, int nchunk, int nsrc)
 {
+    static_assert(C % 4 == 0, "whole 16-byte loads per record");
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double sum = 0.0, comp = 0.0;
     for (int ch = 0; ch < nchunk; ch++) {
-        const float *p = Lm + ((size_t)ch * N + n) * C;
+        const float4 *p = reinterpret_cast<const float4 *>(Lm + ((size_t)ch * N + n) * C);
+        float v[C];
+#pragma unroll
+        for (int q = 0; q < C / 4; q++) {
+            const float4 w = p[q];
+            v[4 * q] = w.x;
+            v[4 * q + 1] = w.y;
+            v[4 * q + 2] = w.z;
+            v[4 * q + 3] = w.w;
+        }
+        const int m = nsrc - ch * C;  // padding sources of the last chunk are left out
+#pragma unroll
         for (int c = 0; c < C; c++) {
-            if (ch * C + c >= nsrc) break;
-            const double y = (double)p[c] - comp;
-            const double tt = sum + y;
-            comp = (tt - sum) - y;
-            sum = tt;
+            if (c < m) {
+                const double y = (double)v[c] - comp;
+                const double tt = sum + y;
+                comp = (tt - sum) - y;
+                sum = tt;
+            }
         }
     }
     const double v = vp[n];
@@ -1703,30 +735,14 @@ __global__ void tile8_copy(const float *__restrict__ f, float *__restrict__ ft, 
 
 namespace pfast {
 
-// Sweep kernel variants (schedule, tile rows x columns, batch/prefetch depth).
-// Variant 0 is the default; the others stay selectable for measurement.
-static const SweepVariant kVariants[] = {
-    {0, 8, 16, 6, "column-window 8x16 L6"},
-    {1, 8, 8, 2, "slab-ring 8x8 L2"},
-    {0, 8, 8, 4, "column-window 8x8 L4"},
-    {0, 8, 8, 2, "column-window 8x8 L2"},
-    {0, 8, 4, 4, "column-window 8x4 L4"},
-    {2, 8, 8, 12, "loader-wave 8x8 A12/8"},
-    {0, 4, 4, 6, "column-window 4x4 L6"},
-    {0, 4, 4, 4, "column-window 4x4 L4"},
-    {0, 4, 8, 6, "column-window 4x8 L6"},
-    {0, 8, 4, 6, "column-window 8x4 L6"},
-    {0, 8, 8, 6, "column-window 8x8 L6"},
-    {0, 16, 8, 6, "column-window 16x8 L6"},
-    {0, 8, 16, 4, "column-window 8x16 L4"},
-    {0, 8, 8, 8, "column-window 8x8 L8"},
-    {3, 8, 8, 2, "vector-lane 8x8 L2"},
-    {3, 8, 8, 4, "vector-lane 8x8 L4"},
-    {3, 8, 16, 2, "vector-lane 8x16 L2"},
-    {0, 8, 32, 6, "column-window 8x32 L6"},
-    {0, 16, 16, 6, "column-window 16x16 L6"},
-    {0, 4, 16, 6, "column-window 4x16 L6"},
-};
+// Sweep kernel variants (tile rows x columns, batch depth), all of the column-window
+// schedule.  Variant 0 is the default; the others stay selectable for measurement.
+#define PFAST_VARIANTS(X)                                                                  \
+    X(8, 8, 4) X(8, 8, 6) X(8, 8, 2) X(8, 8, 8) X(8, 4, 6) X(8, 4, 4) X(4, 8, 6) X(4, 4, 6)  \
+    X(4, 4, 4) X(8, 16, 6) X(8, 16, 4) X(16, 8, 6) X(4, 16, 6)
+#define PFAST_VARIANT_ENTRY(TI, TJ, L) {0, TI, TJ, L, "column-window " #TI "x" #TJ " L" #L},
+static const SweepVariant kVariants[] = {PFAST_VARIANTS(PFAST_VARIANT_ENTRY)};
+#undef PFAST_VARIANT_ENTRY
 
 int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 
@@ -1738,6 +754,7 @@ void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int
                 int s2, int s3, dim3 grid)
 {
     constexpr int C = PFAST_CHUNK;
+    static_assert(TI * TJ * C <= 1024, "one thread per column and source");
     const dim3 block(TI * TJ * C);
     if (adjoint) {
         if (s3 > 0)
@@ -1750,85 +767,21 @@ void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int
         hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2, s3);
     }
 }
-
-// loader-wave kernels: forward lookahead AF steps, adjoint AA steps
-template <int TI, int TJ, int L>
-void launch_vec(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
-                int s2, int s3, dim3 grid)
-{
-    const dim3 block(TI * TJ);
-    if (adjoint) {
-        if (s3 > 0)
-            hipLaunchKernelGGL((adj3d_vec<TI, TJ, L, true>), grid, block, 0, st, a, D, s1, s2);
-        else
-            hipLaunchKernelGGL((adj3d_vec<TI, TJ, L, false>), grid, block, 0, st, a, D, s1, s2);
-    } else if (uniform) {
-        hipLaunchKernelGGL((fwd3d_vec<TI, TJ, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
-    } else {
-        hipLaunchKernelGGL((fwd3d_vec<TI, TJ, L, false>), grid, block, 0, st, a, D, s1, s2, s3);
-    }
-}
-
-template <int TI, int TJ, int AF, int AA>
-void launch_dma(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
-                int s2, int s3, dim3 grid)
-{
-    constexpr int C = PFAST_CHUNK;
-    const dim3 block(TI * TJ * C + 64);
-    if (adjoint) {
-        if (s3 > 0)
-            hipLaunchKernelGGL((adj3d_dma<TI, TJ, C, AA, true>), grid, block, 0, st, a, D, s1, s2);
-        else
-            hipLaunchKernelGGL((adj3d_dma<TI, TJ, C, AA, false>), grid, block, 0, st, a, D, s1, s2);
-    } else if (uniform) {
-        hipLaunchKernelGGL((fwd3d_dma<TI, TJ, C, AF, true>), grid, block, 0, st, a, D, s1, s2, s3);
-    } else {
-        hipLaunchKernelGGL((fwd3d_dma<TI, TJ, C, AF, false>), grid, block, 0, st, a, D, s1, s2, s3);
-    }
-}
 }  // namespace
 
 hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
                           int uniform, int D, int s1, int s2, int s3, int ntiles_diag)
 {
-    constexpr int C = PFAST_CHUNK;
     const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));
-    switch (variant) {
-    case 0: launch_col<8, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 1: {
-        constexpr int TI = 8, TJ = 8, L = 2;
-        const dim3 block(TI * TJ * C);
-        if (adjoint)
-            hipLaunchKernelGGL((adj3d_slab<TI, TJ, C, L>), grid, block, 0, st, a, D, s1, s2, s3);
-        else if (uniform)
-            hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2,
-                               s3);
-        else
-            hipLaunchKernelGGL((fwd3d_slab<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1,
-                               s2, s3);
-        break;
+    int v = 0;
+#define PFAST_VARIANT_CASE(TI, TJ, L)                                               \
+    if (variant == v++) {                                                            \
+        launch_col<TI, TJ, L>(st, a, adjoint, uniform, D, s1, s2, s3, grid);         \
+        return hipGetLastError();                                                    \
     }
-    case 2: launch_col<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 3: launch_col<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 4: launch_col<8, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 5: launch_dma<8, 8, 12, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 6: launch_col<4, 4, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 7: launch_col<4, 4, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 8: launch_col<4, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 9: launch_col<8, 4, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 10: launch_col<8, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 11: launch_col<16, 8, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 12: launch_col<8, 16, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 13: launch_col<8, 8, 8>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 14: launch_vec<8, 8, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 15: launch_vec<8, 8, 4>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 16: launch_vec<8, 16, 2>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 17: launch_col<8, 32, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 18: launch_col<16, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    case 19: launch_col<4, 16, 6>(st, a, adjoint, uniform, D, s1, s2, s3, grid); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    PFAST_VARIANTS(PFAST_VARIANT_CASE)
+#undef PFAST_VARIANT_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk)
@@ -1873,8 +826,8 @@ hipError_t launch_adj_misfit(hipStream_t st, const float *T, const long *pnode,
 hipError_t launch_grad_reduce(hipStream_t st, const float *Lm, const double *vp, double *g,
                               long N, int nchunk, int nsrc)
 {
-    hipLaunchKernelGGL(grad_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, Lm, vp,
-                       g, N, nchunk, PFAST_CHUNK, nsrc);
+    hipLaunchKernelGGL(grad_reduce<PFAST_CHUNK>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       st, Lm, vp, g, N, nchunk, nsrc);
     return hipGetLastError();
 }
 

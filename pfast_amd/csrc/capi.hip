@@ -919,9 +919,11 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     ARGCHK(m && src && srnum && out && nsrc > 0, "bad argument");
     const int nrows = m->nrows, ncols = m->ncols, nz = m->nz;
     const long N = (long)nrows * ncols * nz;
-    // the sweep kernels address one chunk field (C sources) with 32-bit byte offsets
-    ARGCHK(N * PFAST_CHUNK * 4 <= (1L << 29), "grid too large for the batched solver (a field "
-                                              "of one source chunk must stay within 512 MiB)");
+    // the sweep kernels address one chunk's field (C sources) with 32-bit byte offsets;
+    // a level outside the column (up to 64 past either end) must still fall outside it
+    ARGCHK((N + 64L * nrows * ncols) * PFAST_CHUNK * 4 <= (1L << 32),
+           "grid too large for the batched solver (one source chunk's field plus 64 levels "
+           "must stay within 4 GiB)");
     for (int i = 0; i < nsrc; i++) ARGCHK(srnum[i] >= 0, "negative srnum");
     auto *s = new pfast_shots3d();
     s->m = m;
