@@ -17,7 +17,7 @@ struct ChunkRef {
     int srcnode[PFAST_CHUNK];  // linear source node, -1 for padding sources
 };
 
-struct SlabArgs {
+struct SweepArgs {
     const ChunkRef *chunks;  // indexed by chunk id
     const int *active;       // chunk ids still sweeping
     unsigned *changed;    // [nchunk] bit c: source c of the chunk changed this cycle
@@ -46,7 +46,7 @@ struct SweepVariant {
 namespace pfast {
 int sweep_variant_count();
 const SweepVariant &sweep_variant(int v);
-hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
+hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int adjoint,
                           int uniform, int D, int s1, int s2, int s3, int ntiles_diag);
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
 hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,

@@ -26,7 +26,7 @@ struct TileCtx {
 };
 
 template <int TI, int TJ>
-__device__ __forceinline__ TileCtx decode_tile(const SlabArgs &a, int D, int s1, int s2)
+__device__ __forceinline__ TileCtx decode_tile(const SweepArgs &a, int D, int s1, int s2)
 {
     TileCtx t;
     const int slot = blockIdx.x % a.nact_p8;
@@ -132,7 +132,7 @@ struct ColLane {
 };
 
 template <int TI, int TJ, int C>
-__device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SlabArgs &a, const TileCtx &t,
+__device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SweepArgs &a, const TileCtx &t,
                                                             int s1, int s2, int s3)
 {
     constexpr int NT = TI * TJ * C, INFSLOT = 2 * NT;  // INF slot of each parity block
@@ -177,16 +177,16 @@ __device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SlabArgs &a, c
 
 // Tile index of (chunk, tile) in the skip records, or -1 without skipping.
 template <int TI, int TJ>
-__device__ __forceinline__ int skip_index(const SlabArgs &a, const TileCtx &t)
+__device__ __forceinline__ int skip_index(const SweepArgs &a, const TileCtx &t)
 {
     return a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
 }
 
-// The sweep of tile ix in this ordering would change nothing (see SlabArgs): its
+// The sweep of tile ix in this ordering would change nothing (see SweepArgs): its
 // clean sweep is newer than every change of the tile and of the neighbour faces
 // it reads (left tile's x+ face, right's x-, upper's y+, lower's y-).
 template <int TI, int TJ>
-__device__ __forceinline__ bool skip_tile(const SlabArgs &a, const TileCtx &t, int ix)
+__device__ __forceinline__ bool skip_tile(const SweepArgs &a, const TileCtx &t, int ix)
 {
     if (ix < 0) return false;
     const int cl = a.tclean[ix * 8 + a.o];
@@ -211,7 +211,7 @@ __device__ __forceinline__ unsigned col_faces(bool valid, int li, int lj, int TI
 
 // Record the outcome of this tile sweep: `what` = bit 4 (some node changed) | the
 // faces on which a node changed, OR-ed over the block.
-__device__ __forceinline__ void record_tile(const SlabArgs &a, int ix, unsigned what, int *any)
+__device__ __forceinline__ void record_tile(const SweepArgs &a, int ix, unsigned what, int *any)
 {
     if (ix < 0) return;
     unsigned w = 0;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void record_tile(const SlabArgs &a, int ix, unsigned 
 }
 
 template <int TI, int TJ, int C, int L, bool UNI>
-__global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1, int s2, int s3)
+__global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s1, int s2, int s3)
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SlabArgs a, int D, int s1
 // traveltimes are read-only; a negative value marks a fixed receiver node).  Only
 // interior nodes are updated, and every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool S3POS>
-__global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SlabArgs a, int D, int s1, int s2)
+__global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
 {
     constexpr int NT = TI * TJ * C;
     constexpr int s3 = S3POS ? 1 : -1;
@@ -750,7 +750,7 @@ const SweepVariant &sweep_variant(int v) { return kVariants[v]; }
 
 namespace {
 template <int TI, int TJ, int L>
-void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int D, int s1,
+void launch_col(hipStream_t st, const SweepArgs &a, int adjoint, int uniform, int D, int s1,
                 int s2, int s3, dim3 grid)
 {
     constexpr int C = PFAST_CHUNK;
@@ -769,7 +769,7 @@ void launch_col(hipStream_t st, const SlabArgs &a, int adjoint, int uniform, int
 }
 }  // namespace
 
-hipError_t launch_sweep3d(hipStream_t st, const SlabArgs &a, int variant, int adjoint,
+hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int adjoint,
                           int uniform, int D, int s1, int s2, int s3, int ntiles_diag)
 {
     const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));

@@ -678,7 +678,7 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     if (!st) st = ctx->stream;
     const SweepVariant &sv = sweep_variant(ctx->sweep);
     const int nti = (g.nrows + sv.ti - 1) / sv.ti, ntj = (g.ncols + sv.tj - 1) / sv.tj;
-    SlabArgs a{};
+    SweepArgs a{};
     a.chunks = chunks;
     a.changed = changed;
     a.nrows = g.nrows;
@@ -696,7 +696,7 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     a.tmod = skipping ? skip->tmod : nullptr;
     a.tclean = skipping ? skip->tclean : nullptr;
     a.o = o;
-    SlabArgs af = a, aa = a;
+    SweepArgs af = a, aa = a;
     af.active = fwd;
     af.nact = nfwd;
     af.nact_p8 = (nfwd + 7) / 8 * 8;

@@ -40,7 +40,7 @@ struct Error {
 
 // LDS-only workgroup barrier: waits for this wave's LDS traffic, then s_barrier.
 // Outstanding global loads/stores stay in flight (no vmcnt wait), which is what
-// lets the slab prefetch overlap the sweep.
+// lets the batch prefetch overlap the sweep steps.
 __device__ __forceinline__ void lds_barrier()
 {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");

@@ -73,7 +73,7 @@ inline Grid3 grid_of(const pfast_model3d *m)
 // anti-diagonal) on ctx's stream over the chunk ids fwd[0..nfwd) (forward sweeps)
 // and adj[0..nadj) (adjoint sweeps) of the chunk table `chunks`, all of grid g;
 // device arrays.  changed[id] collects, per chunk id, the sources that changed.
-// Exact tile skipping records of one phase (see SlabArgs) and the cycle number that
+// Exact tile skipping records of one phase (see SweepArgs) and the cycle number that
 // makes launch epochs increase; records are indexed by chunk id.
 struct TileSkip {
     int *tmod, *tclean;
