@@ -241,7 +241,7 @@ def main():
     adj_bytes = adj_units * 12.0             # lambda read+write, T read
     fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
     adj_gbs = adj_bytes / (adj_ms * 1e-3) / 1e9
-    kfwd, kadj = ("fwd3d_slab", "adj3d_slab") if "slab" in sweep_name else ("fwd3d_col", "adj3d_col")
+    kfwd, kadj = "fwd3d_col", "adj3d_col"
     dom = kadj if adj_ms >= fwd_ms else kfwd
     log(f"kernels: fwd {fwd_ms:.1f} ms / {fwd_n} launches -> {fwd_gbs:.0f} GB/s; "
         f"adj {adj_ms:.1f} ms / {adj_n} launches -> {adj_gbs:.0f} GB/s")

@@ -1,10 +1,9 @@
-"""CPU check of the batched 3D sweep schedules (pfast_amd/csrc/batch3d.hip).
+"""CPU check of the batched 3D sweep schedule (pfast_amd/csrc/batch3d.hip).
 
-tests/emu/sweep_schedule.cpp replays the kernels' schedules on the CPU -- the
-slab ring (tile anti-diagonal launches, per-thread level skew, LDS ring slots,
-register prefetch, dirty-node write-back) and the column window (register
-windows, parity-buffered exchange of levels m-1 / m+1, halo and INF-slot
-neighbours, batched loads and delayed stores) -- and compares every one of the 8 orderings,
+tests/emu/sweep_schedule.cpp replays the kernels' column-window schedule on the CPU
+(tile anti-diagonal launches, per-thread level skew, register windows,
+parity-buffered exchange of levels m-1 / m+1, halo and INF-slot neighbours,
+batched loads and delayed stores) and compares every one of the 8 orderings,
 bit for bit, with the plain lexicographic sweep in the reference's loop order
 (src/T_pfsm3d.h:131-185 forward, src/T_padjoint3d.h:73-172 adjoint) using the
 same per-node arithmetic.  Partial edge tiles, nz not a multiple of the
@@ -33,31 +32,28 @@ def emu(tmp_path_factory):
     return exe
 
 
-def _run(exe, tmp, mode, shape, tile, T, U0, src, sched="slab"):
+def _run(exe, tmp, mode, shape, tile, T, U0, src):
     nz, nr, nc = shape
-    f = tmp / f"in_{mode}_{sched}_{nz}_{nr}_{nc}_{tile[0]}_{tile[1]}_{tile[2]}.bin"
+    f = tmp / f"in_{mode}_{nz}_{nr}_{nc}_{tile[0]}_{tile[1]}_{tile[2]}.bin"
     with open(f, "wb") as fh:
         fh.write(np.ascontiguousarray(T, np.float32).tobytes())
         fh.write(np.ascontiguousarray(U0, np.float32).tobytes())
         fh.write(np.int32(src).tobytes())
-    r = subprocess.run([str(exe), mode, sched, str(nr), str(nc), str(nz), *map(str, tile), str(f)],
+    r = subprocess.run([str(exe), mode, str(nr), str(nc), str(nz), *map(str, tile), str(f)],
                        capture_output=True, text=True)
     return r.returncode, r.stdout
 
 
-SHAPES = [((11, 19, 13), (8, 8, 2)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 8, 3)),
-          ((7, 5, 23), (8, 4, 1)), ((12, 16, 9), (8, 8, 4))]
-# column window: the batch depth L is even (static step parity inside a batch)
-COL_SHAPES = [((11, 19, 13), (8, 8, 4)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 4, 4)),
+# the batch depth L is even (static step parity inside a batch)
+CASES = [((11, 19, 13), (8, 8, 4)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 4, 4)),
               ((7, 5, 23), (8, 4, 6)), ((12, 16, 9), (8, 8, 4)), ((3, 17, 9), (8, 8, 4)),
               ((10, 14, 11), (4, 4, 6)), ((8, 13, 19), (4, 8, 6)), ((9, 21, 35), (8, 16, 6)),
               ((7, 37, 18), (16, 8, 4))]
-CASES = [(sh, t, "slab") for sh, t in SHAPES] + [(sh, t, "col") for sh, t in COL_SHAPES]
 
 
-@pytest.mark.parametrize("shape,tile,sched", CASES)
+@pytest.mark.parametrize("shape,tile", CASES)
 @pytest.mark.parametrize("where", ["interior", "face"])
-def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched, where):
+def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, where):
     nz, nr, nc = shape
     rng = np.random.default_rng(nz * 1000 + nr * 10 + nc)
     ftime = (25.0 / rng.uniform(1500, 6000, size=shape)).astype(np.float32)
@@ -65,13 +61,13 @@ def test_forward_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched
     sn = int(np.ravel_multi_index(src, shape))
     U0 = np.full(shape, INF, np.float32)
     U0[src] = 0.0
-    rc, out = _run(emu, tmp_path, "fwd", shape, tile, ftime, U0, sn, sched)
+    rc, out = _run(emu, tmp_path, "fwd", shape, tile, ftime, U0, sn)
     assert rc == 0, out
     assert "mismatches 0" in out
 
 
-@pytest.mark.parametrize("shape,tile,sched", CASES)
-def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched):
+@pytest.mark.parametrize("shape,tile", CASES)
+def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile):
     nz, nr, nc = shape
     rng = np.random.default_rng(7 + nz + nr + nc)
     k, i, j = np.meshgrid(np.arange(nz), np.arange(nr), np.arange(nc), indexing="ij")
@@ -86,18 +82,17 @@ def test_adjoint_schedule_equals_lexicographic(emu, tmp_path, shape, tile, sched
     rj = rng.integers(1, nc - 1, nrec)
     T[rk, ri, rj] = -T[rk, ri, rj]
     L0[rk, ri, rj] = rng.normal(0, 50, nrec).astype(np.float32)
-    rc, out = _run(emu, tmp_path, "adj", shape, tile, T, L0, -1, sched)
+    rc, out = _run(emu, tmp_path, "adj", shape, tile, T, L0, -1)
     assert rc == 0, out
     assert "mismatches 0" in out
 
 
-@pytest.mark.parametrize("plant,mode,sched", [("PLANT_NO_SKEW", "fwd", "slab"),
-                                              ("PLANT_RING_DEPTH", "adj", "slab"),
-                                              ("PLANT_RING_DEPTH", "adj", "col"),
-                                              ("PLANT_SWAP_EXCHANGE", "fwd", "col")])
-def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode, sched):
+@pytest.mark.parametrize("plant,mode", [("PLANT_NO_SKEW", "fwd"), ("PLANT_NO_SKEW", "adj"),
+                                        ("PLANT_RING_DEPTH", "adj"),
+                                        ("PLANT_SWAP_EXCHANGE", "fwd")])
+def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode):
     """The comparison is not vacuous: the emulator built with a deliberate
-    schedule mistake (all columns on one level; z-neighbours named by ring order
+    schedule mistake (all columns on one level; z-neighbours named by window order
     instead of absolute depth; upstream and downstream exchange slots swapped)
     reports mismatches."""
     gxx = shutil.which("g++")
@@ -106,7 +101,7 @@ def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode, sched):
     exe = tmp_path / "planted"
     subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", f"-D{plant}", str(SRC), "-o",
                     str(exe)], check=True)
-    shape, tile = (11, 19, 13), (8, 8, 2)
+    shape, tile = (11, 19, 13), (8, 8, 4)
     nz, nr, nc = shape
     rng = np.random.default_rng(3)
     if mode == "fwd":
@@ -123,7 +118,5 @@ def test_planted_schedule_mistake_is_caught(tmp_path, plant, mode, sched):
         T[rk, ri, rj] = -T[rk, ri, rj]
         U0[rk, ri, rj] = rng.normal(0, 50, 30).astype(np.float32)
         src = -1
-    if sched == "col":
-        tile = (8, 8, 4)
-    rc, out = _run(exe, tmp_path, mode, shape, tile, T, U0, src, sched)
+    rc, out = _run(exe, tmp_path, mode, shape, tile, T, U0, src)
     assert rc == 1 and "mismatches 0" not in out, out
