@@ -214,3 +214,18 @@ def test_parent_close_closes_children():
     ctx.close()
     assert sh.h is None and m.h is None and ctx.h is None
     sh.close()
+
+
+def test_chunk_field_bound():
+    """The sweep kernels address a chunk's field with 32-bit byte offsets, levels up to
+    64 past either end of a column included: a grid whose chunk field passes that
+    bound (3400 x 200 x 200 nodes of 8 FP32 sources: 4.43 GB with the margin) is
+    refused when the shot set is created, before any field is allocated."""
+    import pfast_amd
+    nz, nr, nc = 3400, 200, 200
+    ctx = pfast_amd.Context(0)
+    m = pfast_amd.Model3D(ctx, np.full((nz, nr, nc), 2000.0), 25.0, 25.0, 25.0)
+    with pytest.raises(pfast_amd.PfastError, match="too large"):
+        pfast_amd.Shots3D(m, np.array([[nr // 2, nc // 2, 0]]), np.array([0]))
+    m.close()
+    ctx.close()
