@@ -58,10 +58,9 @@ __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
 }
 
 // ============================================================ column sweeps
-// The column window: thread (li,lj,c) keeps its
-// own column of one source in registers -- levels m-1 (updated), m, m+1 (not yet
-// updated) of the level m = p - pi - pj it works on at step p -- so depth needs no
-// exchange at all.  Only the in-plane neighbours do: at every step each thread
+// The column window: thread (li,lj,c) keeps its own column of one source in
+// registers -- levels m-1 (updated), m, m+1 (not yet updated) of the level
+// m = p - pi - pj it works on at step p -- so depth needs no exchange at all.  Only the in-plane neighbours do: at every step each thread
 // publishes two values to LDS, its level m-1 (wanted by the neighbour one step
 // ahead of it, i.e. upstream in the sweep, which now works on level m) and its
 // level m+1 (wanted by the neighbour one step behind, downstream, which now works
@@ -286,12 +285,11 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
         oF = 4u * (k0 * tplane + tix * 64u + (unsigned)(l.li * TJ + l.lj));
         stF = s3 > 0 ? 4u * tplane : 0u - 4u * tplane;
     }
-    // Vector-memory instructions are what the sweep's throughput is bound by (each
-    // costs the address unit the same whatever its lanes hold), so: halo loads run
-    // only in the lanes that use them (a wave without such a lane skips them), and
-    // the cell time, one value per node, is loaded for four levels at once, lane c
-    // of each DPP quad of the node's lanes taking level 4g + c%4; step u broadcasts
-    // it within the quad.
+    // The vector-memory path (L1 misses above all) bounds the sweep's throughput,
+    // so: halo loads run only in the lanes that use them (a wave without such a lane
+    // skips them), and the cell time, one value per node, is loaded for four levels
+    // at once, lane c of each DPP quad of the node's lanes taking level 4g + c%4;
+    // step u broadcasts it within the quad.
     constexpr int FQ = (L + 3) / 4;
     const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
     const unsigned oFc = oF + (unsigned)(l.c & 3) * stF;
