@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (3 times), the "
                          "known-size kernel that scales the HBM byte counters of a profile")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the sweep variant by timing every variant on this shot set "
+                         "first (results are the same under each)")
     ap.add_argument("--no-tile-skip", action="store_true",
                     help="sweep every tile (exact tile skipping off; same results)")
     ap.add_argument("--streams", type=int, default=3,
@@ -206,6 +209,11 @@ def main():
     shots = pfast_amd.Shots3D(model, cfg["src"], cfg["srnum"], cfg["rec"], cfg["nxyz"], cfg["T_obs"])
     ctx.sync()
     log("inputs resident in HBM")
+    if args.autotune:
+        args.sweep, ms = shots.autotune()
+        sweep_name = pfast_amd.sweep_variants()[args.sweep]
+        log("autotune: " + ", ".join(f"{v} {t:.0f} ms" for v, t in ms.items()) +
+            f" -> {sweep_name}")
 
     for w in range(args.warmup):
         t = time.perf_counter()

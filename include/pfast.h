@@ -123,6 +123,14 @@ int pfast_shots3d_destroy(pfast_shots3d *s);
  * gradient/misfit reductions) for every source of the set.  MAXSwp caps the sweep
  * cycles per solve.  Results stay on the device until fetched. */
 int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp);
+/* Sweep-variant tuning (no counterpart in the reference, whose sweeps have one fixed
+ * loop order): times two forward cycles of this shot set under each listed variant
+ * (all when variants is NULL), sets the fastest on the shot set's context and returns
+ * it in *chosen, with each candidate's time in ms[i] when ms is not NULL.  Every
+ * variant gives bit-identical results; the shot set's fields are left mid-solve, so
+ * run it again afterwards. */
+int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants, int *chosen,
+                           double *ms);
 
 /* Fetch results.  grad: N doubles (sum over sources of -lambda/v^3, the
  * reference's global gradient before regularisation).  misfit: sum of EE.

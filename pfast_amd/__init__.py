@@ -253,6 +253,19 @@ class Shots3D:
         check(lib().pfast_shots3d_run(self.h, PFAST_FWD_ADJ if adjoint else PFAST_FWD_ONLY,
                                       int(MAXSwp)))
 
+    def autotune(self, variants=None):
+        """Time two forward cycles of this shot set under each sweep variant (all of
+        them by default), set the fastest on the context and return (variant, {variant:
+        ms}).  Results are the same under every variant; run the shot set afterwards."""
+        cand = None if variants is None else _i32(variants)
+        n = len(sweep_variants()) if cand is None else len(cand)
+        ms = np.zeros(n)
+        chosen = C.c_int(0)
+        check(lib().pfast_shots3d_autotune(self.h, ptr(cand), 0 if cand is None else n,
+                                           C.byref(chosen), ptr(ms)))
+        keys = range(n) if cand is None else cand.tolist()
+        return chosen.value, dict(zip(keys, ms.tolist()))
+
     def gradient(self):
         N = int(np.prod(self.model.shape))
         g = np.empty(N)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1058,6 +1059,49 @@ int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp)
         (rc = pfast::adj_end3d(s)))
         return rc;
     HIPCHK(hipStreamSynchronize(st));
+    return PFAST_OK;
+}
+
+int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants, int *chosen,
+                           double *ms)
+{
+    ARGCHK(s && chosen, "NULL argument");
+    pfast_ctx *ctx = s->m->ctx;
+    const int nall = pfast::sweep_variant_count();
+    std::vector<int> cand;
+    if (variants) {
+        ARGCHK(nvariants > 0, "no candidate variants");
+        for (int i = 0; i < nvariants; i++)
+            ARGCHK(variants[i] >= 0 && variants[i] < nall, "unknown sweep variant");
+        cand.assign(variants, variants + nvariants);
+    } else {
+        for (int v = 0; v < nall; v++) cand.push_back(v);
+    }
+    // every variant computes the same results, so the choice is by time alone: one warm-up
+    // cycle, then two timed forward cycles from the initial field, per candidate
+    const int keep = ctx->sweep;
+    int best = keep;
+    double tbest = 0.0;
+    for (size_t i = 0; i < cand.size(); i++) {
+        ctx->sweep = cand[i];
+        int rc = pfast_shots3d_run(s, PFAST_FWD_ONLY, 1);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!rc) rc = pfast_shots3d_run(s, PFAST_FWD_ONLY, 2);
+        if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PFAST_ERR_HIP;
+        if (rc) {
+            ctx->sweep = keep;
+            return rc;
+        }
+        const double dt =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms) ms[i] = dt;
+        if (i == 0 || dt < tbest) {
+            tbest = dt;
+            best = cand[i];
+        }
+    }
+    ctx->sweep = best;
+    *chosen = best;
     return PFAST_OK;
 }
 

@@ -229,3 +229,29 @@ def test_chunk_field_bound():
         pfast_amd.Shots3D(m, np.array([[nr // 2, nc // 2, 0]]), np.array([0]))
     m.close()
     ctx.close()
+
+
+def test_autotune_picks_fastest_candidate_results_unchanged():
+    """pfast_shots3d_autotune times the listed variants, leaves the fastest set on the
+    context, and the shot set run afterwards gives the same bits as under the default."""
+    import pfast_amd
+    c = cases.case3d(47, shape=(17, 29, 33), spacing=(25.0, 25.0, 25.0), nsrc=11, nrec_per=9)
+    ctx = pfast_amd.Context(0)
+    m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+    sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+    sh.run(adjoint=True, MAXSwp=100)
+    ref = (np.stack([sh.field(s) for s in range(11)]),
+           np.stack([sh.field(s, adjoint=True) for s in range(11)])) + sh.gradient()
+    chosen, ms = sh.autotune([0, 1, 4])
+    print(f"autotune: {ms} -> {chosen}", flush=True)
+    assert sorted(ms) == [0, 1, 4] and all(t > 0 for t in ms.values())
+    assert ms[chosen] == min(ms.values())
+    sh.run(adjoint=True, MAXSwp=100)
+    out = (np.stack([sh.field(s) for s in range(11)]),
+           np.stack([sh.field(s, adjoint=True) for s in range(11)])) + sh.gradient()
+    assert all(np.array_equal(a, b) for a, b in zip(out[:3], ref[:3])) and out[3] == ref[3]
+    with pytest.raises(pfast_amd.PfastError):
+        sh.autotune([len(pfast_amd.sweep_variants())])
+    sh.close()
+    m.close()
+    ctx.close()
