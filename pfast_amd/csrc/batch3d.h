@@ -9,8 +9,9 @@
 // through a table of these, so the chunks of one launch may belong to different
 // shot sets and models of the same grid shape.
 struct ChunkRef {
-    float *field[2];           // [N][C]: [0] traveltimes (negative marks receiver
-                               // nodes during the adjoint), [1] lambda
+    float *field[2];           // [0] traveltimes [N][C] (negative marks receiver nodes
+                               // during the adjoint); [1] adjoint field [N][C] of
+                               // (lambda, T) float pairs
     const float *slow;         // [N] the model's cell time h/v (uniform) or slowness 1/v
     const float *slow8;        // the same values 8x8-tile-major: [nz][nti8][ntj8][8][8]
                                // (tile (ti,tj) row li column lj; edge tiles padded)
@@ -51,17 +52,20 @@ hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int a
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
 hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,
                            int npick, long N, int unmark);
-hipError_t launch_adj_bc(hipStream_t st, float *Lm, const float *T, const long *bnode,
+// The adjoint field LT interleaves (lambda, T) per node and source: 2 floats per
+// traveltime-field value, lambda first.
+hipError_t launch_adj_pack(hipStream_t st, float *LT, const float *T, long N, int nchunk);
+hipError_t launch_adj_bc(hipStream_t st, float *LT, const float *T, const long *bnode,
                          const int *bsrc, const double *bobs, const double *bnrm, int nb,
                          int nrows, int ncols, int nz, double dx, double dy, double dz);
 hipError_t launch_adj_misfit(hipStream_t st, const float *T, const long *pnode,
                              const double *pobs, const long *poff, int nsrc, long N, double *EE);
-hipError_t launch_grad_reduce(hipStream_t st, const float *Lm, const double *vp, double *g,
+hipError_t launch_grad_reduce(hipStream_t st, const float *LT, const double *vp, double *g,
                               long N, int nchunk, int nsrc);
 hipError_t launch_gather_picks(hipStream_t st, const float *T, const long *rnode,
                                const int *rsrc, int nrec, long N, double *out);
-hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int strip_sign,
-                               double *out);
+hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int rec,
+                               int strip_sign, double *out);
 hipError_t launch_cell_time(hipStream_t st, const double *vp, float *f, long N, double h,
                             int uniform);
 // Elements of the 8x8-tile-major copy of a cell-time array, and the copy itself.

@@ -1,10 +1,12 @@
 // batch3d.hip -- batched FP32 3D sweeps (every source of a shot set on one GPU).
 //
 // Layout in HBM: fields are stored per chunk of C sources, node-major with the C
-// sources innermost: F[k][i][j][c] (k = depth, i = row, j = column).  Each chunk
-// names its traveltime and lambda fields and its model's velocity-derived cell
-// time f[k][i][j] (h/v for uniform spacing, else 1/v) through a ChunkRef, so one
-// launch can sweep chunks of several shot sets and models of one grid shape.
+// sources innermost: F[k][i][j][c] (k = depth, i = row, j = column).  The adjoint
+// field pairs every lambda with a copy of its traveltime, LT[k][i][j][c] = (lambda, T),
+// so that one 8-byte load brings both values an adjoint update reads of a neighbour.
+// Each chunk names its traveltime and adjoint fields and its model's velocity-derived
+// cell time f[k][i][j] (h/v for uniform spacing, else 1/v) through a ChunkRef, so
+// one launch can sweep chunks of several shot sets and models of one grid shape.
 //
 // Schedule: the (i,j) plane is cut into TI x TJ column tiles that run the whole
 // depth.  For one ordering (s3,s1,s2) tile (ti',tj') (coordinates taken in sweep
@@ -86,6 +88,10 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
 __device__ __forceinline__ float bload(rsrc_t r, unsigned off)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ float2 bload2(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 __device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
 {
@@ -399,9 +405,11 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
                 &any_change);
 }
 
-// Adjoint, column window: lambda and traveltime travel together as float2 (the
-// traveltimes are read-only; a negative value marks a fixed receiver node).  Only
-// interior nodes are updated, and every neighbour they read is inside the domain.
+// Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
+// node's pair in one 8-byte access (the traveltimes are read-only; a negative value
+// marks a fixed receiver node), and a changed node stores its lambda alone.  Record
+// offsets are twice the traveltime field's.  Only interior nodes are updated, and
+// every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
 {
@@ -421,15 +429,18 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
-    const unsigned fbytes = 4u * plane * (unsigned)nz * C;
-    const rsrc_t RL = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
-    const rsrc_t RT = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
+    const unsigned fbytes = 8u * plane * (unsigned)nz * C;
+    const rsrc_t RLT = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
+    // 8-byte records: the traveltime field's offsets doubled, mod 2^32 (the shot set's
+    // size bound, 8 (N + 64 nrows ncols) C <= 2^32, keeps a level outside the column
+    // outside the field)
+    const unsigned oI = 2u * l.oI, oHX = 2u * l.oHX, oHY = 2u * l.oHY;
     const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
     const bool col_interior =
         l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
     // interior levels 1 .. nz-2 of interior columns
     const int plo = col_interior ? l.pipj + 1 : 0x3fffffff, phi = l.pipj + nz - 1;
-    const unsigned st = l.step;
+    const unsigned st = 2u * l.step;
     const float wx = a.wx, wy = a.wy, wz = a.wz;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
@@ -439,21 +450,21 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     // halo loads only in the lanes that use them (vector-memory instructions bound
     // the sweep; a wave without a halo lane skips them)
     const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
-    auto ld2 = [&](unsigned o) { return make_float2(bload(RL, o), bload(RT, o)); };
+    auto ld2 = [&](unsigned o) { return bload2(RLT, o); };
     auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
         const unsigned dm = (unsigned)(pb - l.pipj) * st;
 #pragma unroll
-        for (int u = 0; u < L; u++) ow[u] = ld2(l.oI + dm + (unsigned)(u + 2) * st);
+        for (int u = 0; u < L; u++) ow[u] = ld2(oI + dm + (unsigned)(u + 2) * st);
         if (hxuse) {
 #pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = ld2(l.oHX + dm + (unsigned)u * st);
+            for (int u = 0; u < L; u++) hx[u] = ld2(oHX + dm + (unsigned)u * st);
         } else {
 #pragma unroll
             for (int u = 0; u < L; u++) hx[u] = make_float2(0.0f, 0.0f);
         }
         if (hyuse) {
 #pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = ld2(l.oHY + dm + (unsigned)u * st);
+            for (int u = 0; u < L; u++) hy[u] = ld2(oHY + dm + (unsigned)u * st);
         } else {
 #pragma unroll
             for (int u = 0; u < L; u++) hy[u] = make_float2(0.0f, 0.0f);
@@ -461,8 +472,8 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     };
     const int m0 = -l.pipj;
     float2 vprev = make_float2(0.0f, 0.0f);
-    float2 vcur = l.valid && m0 == 0 ? ld2(l.oI) : make_float2(0.0f, 0.0f);
-    float2 vnext = l.valid && m0 >= -1 ? ld2(l.oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
+    float2 vcur = l.valid && m0 == 0 ? ld2(oI) : make_float2(0.0f, 0.0f);
+    float2 vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
     float2 qa[3][L], qb[3][L];
     load_batch(0, qa[0], qa[1], qa[2]);
     __syncthreads();
@@ -485,7 +496,7 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
             const unsigned d = (unsigned)(pbo - L - l.pipj) * st;
 #pragma unroll
             for (int u = 0; u < L; u++)
-                if (chg[u]) bstore(RL, l.oI + d + (unsigned)u * st, sval[u]);
+                if (chg[u]) bstore(RLT, oI + d + (unsigned)u * st, sval[u]);
         }
         load_batch(pbo + L, nq[0], nq[1], nq[2]);
 #pragma unroll
@@ -542,7 +553,7 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
         const unsigned d = (unsigned)(pb - L - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++)
-            if (chg[u]) bstore(RL, l.oI + d + (unsigned)u * st, sval[u]);
+            if (chg[u]) bstore(RLT, oI + d + (unsigned)u * st, sval[u]);
     }
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
@@ -577,6 +588,15 @@ __global__ void adj_mark(float *__restrict__ T, const long *__restrict__ pnode,
     T[a] = -fabsf(T[a]);
 }
 
+// Start of an adjoint: LT = (0, T) for every node and source of every chunk (T
+// already carries the receiver marks).  One thread per (chunk, node, source) value.
+__global__ void adj_pack(float2 *__restrict__ LT, const float *__restrict__ T, long total)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    LT[e] = make_float2(0.0f, T[e]);
+}
+
 __global__ void adj_unmark(float *__restrict__ T, const long *__restrict__ pnode,
                            const int *__restrict__ psrc, int npick, long N, int C)
 {
@@ -590,8 +610,9 @@ __global__ void adj_unmark(float *__restrict__ T, const long *__restrict__ pnode
 // Receiver boundary condition lambda = r / (n . grad T) with the reference's one-sided
 // differences (src/T_padjoint3d.h:102-122), FP64 from the FP32 traveltimes.
 // One thread per boundary entry (the last picked receiver of each node, receivers
-// at the source excluded -- resolved on the host).
-__global__ void adj_bc(float *__restrict__ Lm, const float *__restrict__ T,
+// at the source excluded -- resolved on the host).  lambda goes to the .x of the
+// interleaved (lambda, T) field.
+__global__ void adj_bc(float2 *__restrict__ LT, const float *__restrict__ T,
                        const long *__restrict__ bnode, const int *__restrict__ bsrc,
                        const double *__restrict__ bobs, const double *__restrict__ bnrm, int nb,
                        int nrows, int ncols, int nz, double dx, double dy, double dz, int C)
@@ -612,7 +633,7 @@ __global__ void adj_bc(float *__restrict__ Lm, const float *__restrict__ T,
     const double dot = bnrm[e * 3 + 1] * Tx + bnrm[e * 3 + 0] * Ty + bnrm[e * 3 + 2] * Tz;
     double lam = 0.0;
     if (fabs(r) > 1e-9 && fabs(dot) > 1e-6) lam = r / dot;
-    Lm[(size_t)(s / C) * N * C + (size_t)n * C + (s % C)] = (float)lam;
+    LT[(size_t)(s / C) * N * C + (size_t)n * C + (s % C)].x = (float)lam;
 }
 
 // Per-source misfit EE = sum 1/2 (T_obs - T)^2 over picked receivers, Kahan-summed
@@ -645,26 +666,24 @@ __global__ void adj_misfit(const float *__restrict__ T, const long *__restrict__
 }
 
 // grad[n] = -(sum_s lambda_s[n]) / vp[n]^3, Kahan-summed in FP64 over sources in
-// source order.  A node's C values of a chunk are one 16-byte-aligned record, read
-// with 16-byte loads.
+// source order.  A node's C (lambda, T) pairs of a chunk are one 16-byte-aligned
+// record, read with 16-byte loads (two pairs each).
 template <int C>
-__global__ void grad_reduce(const float *__restrict__ Lm, const double *__restrict__ vp,
+__global__ void grad_reduce(const float2 *__restrict__ LT, const double *__restrict__ vp,
                             double *__restrict__ g, long N, int nchunk, int nsrc)
 {
-    static_assert(C % 4 == 0, "whole 16-byte loads per record");
+    static_assert(C % 2 == 0, "whole 16-byte loads per record");
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double sum = 0.0, comp = 0.0;
     for (int ch = 0; ch < nchunk; ch++) {
-        const float4 *p = reinterpret_cast<const float4 *>(Lm + ((size_t)ch * N + n) * C);
+        const float4 *p = reinterpret_cast<const float4 *>(LT + ((size_t)ch * N + n) * C);
         float v[C];
 #pragma unroll
-        for (int q = 0; q < C / 4; q++) {
+        for (int q = 0; q < C / 2; q++) {
             const float4 w = p[q];
-            v[4 * q] = w.x;
-            v[4 * q + 1] = w.y;
-            v[4 * q + 2] = w.z;
-            v[4 * q + 3] = w.w;
+            v[2 * q] = w.x;
+            v[2 * q + 1] = w.z;
         }
         const int m = nsrc - ch * C;  // padding sources of the last chunk are left out
 #pragma unroll
@@ -691,14 +710,15 @@ __global__ void gather_picks(const float *__restrict__ T, const long *__restrict
     out[e] = (double)fabsf(T[((size_t)(s / C) * N + rnode[e]) * C + (s % C)]);
 }
 
-// Widen one source's field to FP64.  Traveltimes (strip_sign) may carry the
-// receiver marks of an adjoint in progress; adjoint fields are signed.
-__global__ void gather_field(const float *__restrict__ F, int s, long N, int C, int strip_sign,
-                             double *__restrict__ out)
+// Widen one source's field to FP64: a traveltime field (rec = 1) or the lambdas of
+// the interleaved (lambda, T) field (rec = 2).  Traveltimes (strip_sign) may carry
+// the receiver marks of an adjoint in progress; adjoint fields are signed.
+__global__ void gather_field(const float *__restrict__ F, int s, long N, int C, int rec,
+                             int strip_sign, double *__restrict__ out)
 {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const float v = F[((size_t)(s / C) * N + n) * C + (s % C)];
+    const float v = F[(((size_t)(s / C) * N + n) * C + (s % C)) * rec];
     out[n] = (double)(strip_sign ? fabsf(v) : v);
 }
 
@@ -803,12 +823,21 @@ hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const in
     return hipGetLastError();
 }
 
-hipError_t launch_adj_bc(hipStream_t st, float *Lm, const float *T, const long *bnode,
+hipError_t launch_adj_pack(hipStream_t st, float *LT, const float *T, long N, int nchunk)
+{
+    const long tot = N * nchunk * PFAST_CHUNK;
+    hipLaunchKernelGGL(adj_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<float2 *>(LT), T, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_bc(hipStream_t st, float *LT, const float *T, const long *bnode,
                          const int *bsrc, const double *bobs, const double *bnrm, int nb,
                          int nrows, int ncols, int nz, double dx, double dy, double dz)
 {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_bc, dim3((nb + 255) / 256), dim3(256), 0, st, Lm, T, bnode, bsrc, bobs,
+    hipLaunchKernelGGL(adj_bc, dim3((nb + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<float2 *>(LT), T, bnode, bsrc, bobs,
                        bnrm, nb, nrows, ncols, nz, dx, dy, dz, PFAST_CHUNK);
     return hipGetLastError();
 }
@@ -821,11 +850,11 @@ hipError_t launch_adj_misfit(hipStream_t st, const float *T, const long *pnode,
     return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(hipStream_t st, const float *Lm, const double *vp, double *g,
+hipError_t launch_grad_reduce(hipStream_t st, const float *LT, const double *vp, double *g,
                               long N, int nchunk, int nsrc)
 {
     hipLaunchKernelGGL(grad_reduce<PFAST_CHUNK>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                       st, Lm, vp, g, N, nchunk, nsrc);
+                       st, reinterpret_cast<const float2 *>(LT), vp, g, N, nchunk, nsrc);
     return hipGetLastError();
 }
 
@@ -838,11 +867,11 @@ hipError_t launch_gather_picks(hipStream_t st, const float *T, const long *rnode
     return hipGetLastError();
 }
 
-hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int strip_sign,
-                               double *out)
+hipError_t launch_gather_field(hipStream_t st, const float *F, int s, long N, int rec,
+                               int strip_sign, double *out)
 {
     hipLaunchKernelGGL(gather_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, s, N,
-                       PFAST_CHUNK, strip_sign, out);
+                       PFAST_CHUNK, rec, strip_sign, out);
     return hipGetLastError();
 }
 

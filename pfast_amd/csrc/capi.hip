@@ -749,10 +749,12 @@ int adj_begin3d(pfast_shots3d *s)
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols * m->nz;
     HIPCHK(launch_adj_misfit(st, s->T, s->pnode, s->pobs, s->poff, s->nsrc, N, s->EE));
-    HIPCHK(hipMemsetAsync(s->L, 0, (size_t)s->nchunk * N * PFAST_CHUNK * sizeof(float), st));
+    // marks first, so that the traveltime copies of the (lambda, T) field carry them;
+    // the boundary values read |T|
+    HIPCHK(launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 0));
+    HIPCHK(launch_adj_pack(st, s->L, s->T, N, s->nchunk));
     HIPCHK(launch_adj_bc(st, s->L, s->T, s->bnode, s->bsrc, s->bobs, s->bnrm, s->nb, m->nrows,
                          m->ncols, m->nz, m->dx, m->dy, m->dz));
-    HIPCHK(launch_adj_mark(st, s->T, s->pnode, s->psrc, s->npick, N, 0));
     return PFAST_OK;
 }
 
@@ -925,6 +927,10 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     ARGCHK((N + 64L * nrows * ncols) * PFAST_CHUNK * 4 <= (1L << 32),
            "grid too large for the batched solver (one source chunk's field plus 64 levels "
            "must stay within 4 GiB)");
+    // the adjoint field holds (lambda, T) pairs: 8 bytes per node and source
+    ARGCHK(!(rec && T_obs && nxyz) || (N + 64L * nrows * ncols) * PFAST_CHUNK * 8 <= (1L << 32),
+           "grid too large for the batched adjoint (one source chunk's (lambda, T) field plus "
+           "64 levels must stay within 4 GiB)");
     for (int i = 0; i < nsrc; i++) ARGCHK(srnum[i] >= 0, "negative srnum");
     auto *s = new pfast_shots3d();
     s->m = m;
@@ -1011,7 +1017,7 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
         }
         s->npick = (int)pnode.size();
         s->nb = (int)bnode.size();
-        if ((rc = pfast::dalloc(m->ctx, &s->L, fsz)) || (rc = pfast::dupload(m->ctx, &s->pnode, pnode.data(), pnode.size())) ||
+        if ((rc = pfast::dalloc(m->ctx, &s->L, 2 * fsz)) || (rc = pfast::dupload(m->ctx, &s->pnode, pnode.data(), pnode.size())) ||
             (rc = pfast::dupload(m->ctx, &s->psrc, psrc.data(), psrc.size())) ||
             (rc = pfast::dupload(m->ctx, &s->pobs, pobs.data(), pobs.size())) ||
             (rc = pfast::dupload(m->ctx, &s->poff, poff.data(), poff.size())) ||
@@ -1029,7 +1035,7 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     for (int i = 0; i < s->nchunk; i++) {
         ChunkRef &c = s->hchunks[i];
         c.field[0] = s->T + (size_t)i * N * PFAST_CHUNK;
-        c.field[1] = s->L ? s->L + (size_t)i * N * PFAST_CHUNK : nullptr;
+        c.field[1] = s->L ? s->L + (size_t)i * N * PFAST_CHUNK * 2 : nullptr;
         c.slow = m->f;
         c.slow8 = m->ft;
         for (int k = 0; k < PFAST_CHUNK; k++) c.srcnode[k] = sn[i * PFAST_CHUNK + k];
@@ -1146,7 +1152,8 @@ int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field)
     pfast_model3d *m = s->m;
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols * m->nz;
-    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, which == 0, s->fieldbuf));
+    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, which ? 2 : 1, which == 0,
+                                      s->fieldbuf));
     HIPCHK(hipMemcpyAsync(field, s->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;

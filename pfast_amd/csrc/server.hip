@@ -146,7 +146,9 @@ size_t footprint3d(const pfast::Grid3 &g, int nsrc, long nrec, bool adjoint)
     const size_t N = (size_t)g.nrows * g.ncols * g.nz;
     const size_t nchunk = (nsrc + PFAST_CHUNK - 1) / PFAST_CHUNK;
     const size_t field = round2m(nchunk * N * PFAST_CHUNK * 4);
-    size_t b = field * (adjoint ? 2 : 1) + 2 * round2m(N * 8) +
+    // the adjoint field holds (lambda, T) pairs
+    const size_t afield = adjoint ? round2m(nchunk * N * PFAST_CHUNK * 8) : 0;
+    size_t b = field + afield + 2 * round2m(N * 8) +
                nchunk * (PFAST_CHUNK * 4 + 8 + sizeof(ChunkRef)) + (size_t)nsrc * 8 +
                (size_t)nrec * (8 + 4 + 12 + 8);
     if (adjoint) b += (size_t)nrec * (8 + 4 + 8 + 8 + 4 + 8 + 24 + 24 + 8) + (size_t)nsrc * 8;
@@ -949,7 +951,7 @@ int pfast_server_field(pfast_server *S, long ticket, int src, double *T)
     if (R->s3) {
         const pfast_model3d *m = R->s3->m;
         const long N = (long)m->nrows * m->ncols * m->nz;
-        HIPCHK(pfast::launch_gather_field(S->copy, R->s3->T, src, N, 1, R->s3->fieldbuf));
+        HIPCHK(pfast::launch_gather_field(S->copy, R->s3->T, src, N, 1, 1, R->s3->fieldbuf));
         HIPCHK(hipMemcpyAsync(T, R->s3->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, S->copy));
         HIPCHK(hipStreamSynchronize(S->copy));
         return PFAST_OK;

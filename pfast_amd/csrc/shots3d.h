@@ -27,7 +27,8 @@ struct pfast_shots3d {
     std::vector<int> srnum;  // nsrc
     std::vector<long> ch0;   // nsrc+1 prefix
     int *srcnode = nullptr;  // [nchunk*C]
-    float *T = nullptr, *L = nullptr;
+    float *T = nullptr;      // [nchunk][N][C] traveltimes
+    float *L = nullptr;      // [nchunk][N][C][2] adjoint (lambda, T) pairs
     ChunkRef *chunks = nullptr;          // [nchunk] device table the sweep kernels read
     int *tmod = nullptr, *tclean = nullptr;  // tile skip records, [nchunk][ntile](x8)
     int ntile = 0;
