@@ -228,6 +228,17 @@ def test_chunk_field_bound():
     with pytest.raises(pfast_amd.PfastError, match="too large"):
         pfast_amd.Shots3D(m, np.array([[nr // 2, nc // 2, 0]]), np.array([0]))
     m.close()
+    # the adjoint field holds (lambda, T) pairs, 8 bytes per node and source: 1700 x 200 x
+    # 200 nodes pass for traveltimes alone (2.26 GB with the margin) and are refused for a
+    # shot set with picks (4.52 GB)
+    nz = 1700
+    m = pfast_amd.Model3D(ctx, np.full((nz, nr, nc), 2000.0), 25.0, 25.0, 25.0)
+    src = np.array([[nr // 2, nc // 2, 0]])
+    pfast_amd.Shots3D(m, src, np.array([0])).close()
+    with pytest.raises(pfast_amd.PfastError, match="too large for the batched adjoint"):
+        pfast_amd.Shots3D(m, src, np.array([1]), np.array([[nr // 2, nc // 2 + 4, 0]]),
+                          np.array([[0.0, 0.0, 1.0]]), np.array([0.1]))
+    m.close()
     ctx.close()
 
 
