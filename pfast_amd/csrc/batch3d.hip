@@ -387,14 +387,18 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
             vnext = q[0][u];
         }
     };
+    // steps 0 .. pend-1, in batches of L (the queues alternate); the loop ends after
+    // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
-    for (int pb = 0; pb < pend; pb += 2 * L) {
+    for (int pb = 0;; pb += 2 * L) {
         batch(pb, qa, qb);
+        if (pb + L >= pend) break;
         batch(pb + L, qb, qa);
+        if (pb + 2 * L >= pend) break;
     }
-    {
-        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
-        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+    {  // changed nodes of the last batch, the one holding step pend-1
+        const int pl = (pend - 1) / L * L;
+        const unsigned d = (unsigned)(pl - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++)
             if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
@@ -543,14 +547,18 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
             vnext = q[0][u];
         }
     };
+    // steps 0 .. pend-1, in batches of L (the queues alternate); the loop ends after
+    // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
-    for (int pb = 0; pb < pend; pb += 2 * L) {
+    for (int pb = 0;; pb += 2 * L) {
         batch(pb, qa, qb);
+        if (pb + L >= pend) break;
         batch(pb + L, qb, qa);
+        if (pb + 2 * L >= pend) break;
     }
-    {
-        const int pb = (pend + 2 * L - 1) / (2 * L) * (2 * L);
-        const unsigned d = (unsigned)(pb - L - l.pipj) * st;
+    {  // changed nodes of the last batch, the one holding step pend-1
+        const int pl = (pend - 1) / L * L;
+        const unsigned d = (unsigned)(pl - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++)
             if (chg[u]) bstore(RLT, oI + d + (unsigned)u * st, sval[u]);

@@ -255,16 +255,19 @@ static void col(bool fwd, const float *T, float *G, int src, int o, int TI, int 
                     }
                 }
             };
+            // batches until the one holding step pend-1, the queues alternating
             const int pend = nz + span;
-            int pb = 0;
-            for (; pb < pend; pb += 2 * L) {
+            for (int pb = 0;; pb += 2 * L) {
                 batch(pb, false);
+                if (pb + L >= pend) break;
                 batch(pb + L, true);
+                if (pb + 2 * L >= pend) break;
             }
+            const int pl = (pend - 1) / L * L;  // the last batch
             for (int t = 0; t < NT; t++) {
                 Th &h = th[t];
                 for (int u = 0; u < L; u++)
-                    if (h.chg[u]) stv(h.gi, h.gj, pb - L - h.pipj + u, h.sval[u]);
+                    if (h.chg[u]) stv(h.gi, h.gj, pl - h.pipj + u, h.sval[u]);
             }
         }
     }
