@@ -125,10 +125,11 @@ int pfast_shots3d_destroy(pfast_shots3d *s);
 int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp);
 /* Sweep-variant tuning (no counterpart in the reference, whose sweeps have one fixed
  * loop order): times two forward cycles of this shot set under each listed variant
- * (all when variants is NULL), sets the fastest on the shot set's context and returns
- * it in *chosen, with each candidate's time in ms[i] when ms is not NULL.  Every
- * variant gives bit-identical results; the shot set's fields are left mid-solve, so
- * run it again afterwards. */
+ * (all when variants is NULL), and two adjoint cycles after them when the set has
+ * picks, sets the fastest on the shot set's context and returns it in *chosen, with
+ * each candidate's time in ms[i] when ms is not NULL.  Every variant gives
+ * bit-identical results; the shot set's fields are left mid-solve, so run it again
+ * afterwards. */
 int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants, int *chosen,
                            double *ms);
 

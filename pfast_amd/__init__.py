@@ -254,9 +254,10 @@ class Shots3D:
                                       int(MAXSwp)))
 
     def autotune(self, variants=None):
-        """Time two forward cycles of this shot set under each sweep variant (all of
-        them by default), set the fastest on the context and return (variant, {variant:
-        ms}).  Results are the same under every variant; run the shot set afterwards."""
+        """Time two forward cycles (and two adjoint cycles, when the shot set has picks)
+        of this shot set under each sweep variant (all of them by default), set the
+        fastest on the context and return (variant, {variant: ms}).  Results are the
+        same under every variant; run the shot set afterwards."""
         cand = None if variants is None else _i32(variants)
         n = len(sweep_variants()) if cand is None else len(cand)
         ms = np.zeros(n)

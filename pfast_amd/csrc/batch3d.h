@@ -38,9 +38,9 @@ struct SweepArgs {
 };
 
 // A sweep kernel variant: schedule (0: column window, the only one), tile rows x
-// columns, batch depth.
+// columns, batch depth of the forward and of the adjoint kernel.
 struct SweepVariant {
-    int kind, ti, tj, depth;
+    int kind, ti, tj, depth, adepth;
     const char *name;
 };
 

@@ -761,12 +761,17 @@ __global__ void tile8_copy(const float *__restrict__ f, float *__restrict__ ft, 
 
 namespace pfast {
 
-// Sweep kernel variants (tile rows x columns, batch depth), all of the column-window
-// schedule.  Variant 0 is the default; the others stay selectable for measurement.
-#define PFAST_VARIANTS(X)                                                                  \
-    X(8, 8, 4) X(8, 8, 6) X(8, 8, 2) X(8, 8, 8) X(8, 4, 6) X(8, 4, 4) X(4, 8, 6) X(4, 4, 6)  \
-    X(4, 4, 4) X(8, 16, 6) X(8, 16, 4) X(16, 8, 6) X(4, 16, 6)
-#define PFAST_VARIANT_ENTRY(TI, TJ, L) {0, TI, TJ, L, "column-window " #TI "x" #TJ " L" #L},
+// Sweep kernel variants (tile rows x columns, batch depth of the forward / adjoint
+// kernel), all of the column-window schedule.  Variant 0 is the default; the others
+// stay selectable for measurement.  The adjoint keeps a (lambda, T) pair per queued
+// value, so a shallower batch keeps its register count at full occupancy (8x8 tiles:
+// 60 VGPRs at L2 against 86 at L4, 8 against 5 waves per SIMD).
+#define PFAST_VARIANTS(X)                                                                 \
+    X(8, 8, 4, 2) X(8, 8, 4, 4) X(8, 8, 6, 6) X(8, 8, 2, 2) X(8, 8, 8, 8) X(8, 8, 6, 2)     \
+    X(8, 4, 6, 6) X(8, 4, 4, 4) X(4, 8, 6, 6) X(4, 4, 6, 6) X(4, 4, 4, 4) X(8, 16, 6, 6)   \
+    X(8, 16, 4, 4) X(16, 8, 6, 6) X(4, 16, 6, 6)
+#define PFAST_VARIANT_ENTRY(TI, TJ, L, LA) \
+    {0, TI, TJ, L, LA, "column-window " #TI "x" #TJ " L" #L "/" #LA},
 static const SweepVariant kVariants[] = {PFAST_VARIANTS(PFAST_VARIANT_ENTRY)};
 #undef PFAST_VARIANT_ENTRY
 
@@ -775,7 +780,7 @@ int sweep_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0]
 const SweepVariant &sweep_variant(int v) { return kVariants[v]; }
 
 namespace {
-template <int TI, int TJ, int L>
+template <int TI, int TJ, int L, int LA>
 void launch_col(hipStream_t st, const SweepArgs &a, int adjoint, int uniform, int D, int s1,
                 int s2, int s3, dim3 grid)
 {
@@ -784,9 +789,9 @@ void launch_col(hipStream_t st, const SweepArgs &a, int adjoint, int uniform, in
     const dim3 block(TI * TJ * C);
     if (adjoint) {
         if (s3 > 0)
-            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2);
+            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, true>), grid, block, 0, st, a, D, s1, s2);
         else
-            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, L, false>), grid, block, 0, st, a, D, s1, s2);
+            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, false>), grid, block, 0, st, a, D, s1, s2);
     } else if (uniform) {
         hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
     } else {
@@ -800,9 +805,9 @@ hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int a
 {
     const dim3 grid((unsigned)(ntiles_diag * a.nact_p8));
     int v = 0;
-#define PFAST_VARIANT_CASE(TI, TJ, L)                                               \
+#define PFAST_VARIANT_CASE(TI, TJ, L, LA)                                           \
     if (variant == v++) {                                                            \
-        launch_col<TI, TJ, L>(st, a, adjoint, uniform, D, s1, s2, s3, grid);         \
+        launch_col<TI, TJ, L, LA>(st, a, adjoint, uniform, D, s1, s2, s3, grid);     \
         return hipGetLastError();                                                    \
     }
     PFAST_VARIANTS(PFAST_VARIANT_CASE)

@@ -1084,7 +1084,9 @@ int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants,
         for (int v = 0; v < nall; v++) cand.push_back(v);
     }
     // every variant computes the same results, so the choice is by time alone: one warm-up
-    // cycle, then two timed forward cycles from the initial field, per candidate
+    // cycle, then two timed forward cycles from the initial field and, with picks, two
+    // timed adjoint cycles on that field, per candidate (the variants set the forward and
+    // the adjoint batch depth apart)
     const int keep = ctx->sweep;
     int best = keep;
     double tbest = 0.0;
@@ -1097,6 +1099,16 @@ int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants,
         if (rc) {
             ctx->sweep = keep;
             return rc;
+        }
+        if (s->have_adj_data) {
+            rc = pfast::adj_begin3d(s);
+            if (!rc) rc = run_sweeps3d(s, true, 2, s->acyc);
+            if (!rc) rc = pfast::adj_end3d(s);
+            if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PFAST_ERR_HIP;
+            if (rc) {
+                ctx->sweep = keep;
+                return rc;
+            }
         }
         const double dt =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
