@@ -136,6 +136,63 @@ def serving(args):
     return res
 
 
+def tomo2d(args):
+    """The 2D workloads of the survey (101x401 tomography, 60 sources x 200 receivers;
+    151x601 joint transmission + reflection, 80 sources x 300 receivers, one reflector):
+    one batched FP32 gradient evaluation (every source's forward and adjoints and the
+    FP64 gradient reduction) on the starting model, picks from the true model's own
+    FP32 forward, against the plain FP64 form of one source."""
+    import numpy as np
+    import pfast_amd
+    from pfast_amd import synth
+    res = {}
+    ctx = pfast_amd.Context(0)
+    for kind, name in ((2, "tomography 101x401, 60 sources x 200 receivers"),
+                       (3, "joint T+R 151x601, 80 sources x 300 receivers, 1 reflector")):
+        cfg = synth.config2d(kind)
+        nsrc, nrec, R = cfg["src"].shape[0], int(cfg["srnum"][0]), cfg["NumofR"]
+        rec, h = cfg["rec"], cfg["h"]
+        mt = pfast_amd.Model2D(ctx, cfg["vp"], h, h, cfg["Z"])
+        st = pfast_amd.Shots2D(mt, cfg["src"], cfg["srnum"], rec, cfg["nxy"])
+        st.run(adjoint=False, MAXSwp=500)
+        tt, tr = np.empty(nsrc * nrec), np.empty((nsrc * nrec, max(1, R)))
+        for s in range(nsrc):
+            rows = slice(s * nrec, (s + 1) * nrec)
+            tt[rows] = st.field(s, "T")[rec[rows, 0], rec[rows, 1]]
+            if R:
+                tr[rows, 0] = st.field(s, "TTu", 0)[rec[rows, 0], rec[rows, 1]]
+        st.close()
+        mt.close()
+        m0 = pfast_amd.Model2D(ctx, cfg["vp0"], h, h, cfg["Z"])
+        sh = pfast_amd.Shots2D(m0, cfg["src"], cfg["srnum"], rec, cfg["nxy"],
+                               *((tt, tr) if R else (tt,)))
+        sh.run(adjoint=True, MAXSwp=500)  # warm-up
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sh.run(adjoint=True, MAXSwp=500)
+            sh.gradient(0.5)
+            best = min(best, time.perf_counter() - t0)
+        sh.close()
+        m0.close()
+        t0 = time.perf_counter()
+        r = pfast_amd.RT_fsm2d(cfg["vp0"], cfg["Z"], cfg["src"], 0, R, 500, h, h)
+        pfast_amd.Tadjoint_fsm2d(r["T"], tt, rec, cfg["nxy"], cfg["src"], cfg["srnum"], 0, 500,
+                                 h, h)
+        if R:
+            pfast_amd.Radjoint_fsm2d(r["TTu"], tr, rec, cfg["nxy"], cfg["src"], cfg["srnum"], 0,
+                                     1, 500, h, h)
+        plain = time.perf_counter() - t0
+        res[name] = {"gradient_ms": round(best * 1e3, 2),
+                     "source_solves_per_s": round(nsrc / best, 1),
+                     "plain_fp64_one_source_ms": round(plain * 1e3, 2),
+                     "ratio_vs_plain_fp64": round(nsrc / best * plain, 1)}
+        log(f"2D {name}: {best * 1e3:.1f} ms per gradient ({nsrc / best:.0f} source-solves/s), "
+            f"plain FP64 {plain * 1e3:.1f} ms per source")
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +225,8 @@ def main():
                     help="arrival window of the serving measurement")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-2d", action="store_true",
+                    help="skip the 2D tomography and joint T+R gradient timings")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -317,6 +376,12 @@ def main():
     shots.close()
     model.close()
     ctx.close()
+    if not args.no_2d:
+        try:
+            out["tomo2d"] = tomo2d(args)
+        except Exception as exc:  # the headline line must still be printed
+            log(f"2D measurement failed: {exc!r}")
+            out["tomo2d"] = {"error": repr(exc)}
     if not args.no_serve:
         try:
             out["serving"] = serving(args)

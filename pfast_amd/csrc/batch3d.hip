@@ -76,32 +76,6 @@ __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
 // L steps one vmcnt(0) wait, then the stores of the previous batch's changed nodes
 // and the loads of the next batch, so every load has a whole batch to arrive.
 
-// Raw buffer access: 32-bit byte offsets from a uniform base, and an offset at or
-// past num_records (a level outside the column, where the unsigned offset wraps)
-// reads 0 without touching memory and never faults.
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
-                                             0x00020000);
-}
-__device__ __forceinline__ float bload(rsrc_t r, unsigned off)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-__device__ __forceinline__ float2 bload2(rsrc_t r, unsigned off)
-{
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
-}
-
-// vmcnt(0) (expcnt and lgkmcnt left at their maxima).  Issued through the builtin
-// so that the compiler's own wait insertion knows nothing is outstanding after it.
-__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
 // Lane Q of this lane's quad (DPP quad_perm [Q,Q,Q,Q]).  The intrinsic is convergent,
 // so it stays where every lane of the quad is active.
 template <int Q>

@@ -38,6 +38,44 @@ struct Error {
         }                                                                                  \
     } while (0)
 
+// Raw buffer access: 32-bit byte offsets from a uniform base, and an offset at or
+// past num_records (a position outside the array, e.g. where an unsigned offset
+// wraps below the base) reads 0 without touching memory and never faults.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ float bload(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ float2 bload2(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double bload_d(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ int bload_i(rsrc_t r, unsigned off)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ unsigned bload_u8(rsrc_t r, unsigned off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
+// vmcnt(0) (expcnt and lgkmcnt left at their maxima).  Issued through the builtin
+// so that the compiler's own wait insertion knows nothing is outstanding after it.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // LDS-only workgroup barrier: waits for this wave's LDS traffic, then s_barrier.
 // Outstanding global loads/stores stay in flight (no vmcnt wait), which is what
 // lets the batch prefetch overlap the sweep steps.

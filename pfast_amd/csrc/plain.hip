@@ -169,6 +169,251 @@ struct Fast32 {
     __device__ static real slow(const double *vp, long n) { return (float)(1.0 / vp[n]); }
 };
 
+// ---------------------------------------------------- 2D FP32 row window
+// One ordering of the batched FP32 form without a memory round trip per plane.
+// Thread t owns sweep-ordered row t and walks it: at step p it updates walk column
+// w = p - t (interior forms: the t-th interior row, w = p - t + 1), so node (t, w) is
+// updated at step t + w, the plane of the plane loop, with the same operands and
+// arithmetic: the same results bit for bit.  A thread keeps columns w-1 (updated),
+// w and w+1 (not yet) of its row in registers.  The row behind it in sweep order is
+// at column w+1 and holds this column's updated value as its "w-1"; the row ahead is
+// at w-1 and holds the old value as its "w+1": every step each thread publishes those
+// two to LDS (double-buffered by step parity, one barrier per step).  Memory traffic
+// is batched over W2L steps: one vmcnt(0) wait, the stores of the previous batch's
+// changed nodes, then the loads of the next batch (own row two columns ahead and the
+// node inputs), so the steps themselves touch registers and LDS only.  The loads are
+// raw buffer loads issued for every column of the batch: one outside the row or the
+// grid reads 0 or a neighbouring row's value, never faults, and is never used (edge
+// columns and rows take the reference's one-sided rules, and only in-range nodes are
+// updated).  Needs one thread per row (blockDim.x == W2NT >= rows).
+constexpr int W2NT = 256, W2L = 8;
+
+__device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ U, int si, int sj,
+                             int mode, const double *__restrict__ Zr, const int *__restrict__ k2vec,
+                             const Grid2 &g, int s1, int s2)
+{
+    constexpr int NT = W2NT, L = W2L, PB = 2 * NT + 1, INFSLOT = 2 * NT;
+    __shared__ float pub[2 * PB];
+    const int nrows = g.nrows, ncols = g.ncols, t = threadIdx.x;
+    const float INF = PFAST_INF_F;
+    const bool valid = t < nrows;
+    const int i = s1 > 0 ? t : nrows - 1 - t;
+    const long rb = valid ? (long)i * ncols : 0;
+    const int rbehind = valid && t > 0 ? t - 1 : INFSLOT;
+    const int rahead = t + 1 < nrows ? NT + t + 1 : INFSLOT;
+    const bool top = s1 > 0 ? t == 0 : t == nrows - 1;     // i - 1 < 0
+    const bool bottom = s1 > 0 ? t == nrows - 1 : t == 0;  // i + 1 >= nrows
+    const float fdx = (float)g.dx, fdz = (float)g.dz, fwx = (float)g.wx, fwz = (float)g.wz;
+    const double zlim = (i + 1) * g.dz;
+    const unsigned N = (unsigned)nrows * ncols;
+    const rsrc_t RU = make_rsrc(U, 4u * N), RV = make_rsrc(vp, 8u * N);
+    const rsrc_t RZ = make_rsrc(Zr, mode == 1 ? 24u * ncols : 0u);
+    auto col = [&](int w) { return s2 > 0 ? w : ncols - 1 - w; };
+    auto in = [&](int w) { return valid && w >= 0 && w < ncols; };
+    auto ld = [&](int w) { return bload(RU, 4u * (unsigned)(rb + col(w))); };
+    // raw inputs of the batch's nodes: own row two columns ahead, velocity, and the
+    // reflector depth of the column (mode 1).  The reflection rows of mode 2 (k2vec)
+    // may live in LDS and are read at the batch start through their own pointer.
+    struct Q {
+        float u[L];
+        double v[L], z[L];
+    };
+    auto load_batch = [&](int pb, Q &q) {
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int w = pb + u - t;
+            const int j = col(w);
+            q.u[u] = ld(w + 2);
+            q.v[u] = bload_d(RV, 8u * (unsigned)(rb + j));
+            if (mode == 1) q.z[u] = bload_d(RZ, 24u * (unsigned)j);
+        }
+    };
+    float prev = INF, cur = ld(-t), next = ld(1 - t);
+    Q qa, qb;
+    load_batch(0, qa);
+    if (t == 0) pub[INFSLOT] = pub[PB + INFSLOT] = INF;
+    __syncthreads();
+    bool changed = false;
+    float sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = 0.0f;
+        chg[u] = false;
+    }
+    auto store_batch = [&](int pb) {  // changed nodes of the batch starting at step pb
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore(RU, 4u * (unsigned)(rb + col(pb + u - t)), sval[u]);
+    };
+    auto batch = [&](int pb, const Q &q, Q &nq) {
+        wait_vm_all();
+        // node flags (src/RT_pfsm2d.h:174, :309) and slowness from the arrived inputs,
+        // before this batch's stores and loads are in flight
+        float sl[L];
+        bool upd[L];
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int w = pb + u - t, j = col(w);
+            sl[u] = (float)(1.0 / q.v[u]);
+            const bool notsrc = !(i == si && j == sj);
+            if (mode == 2) {
+                const int jc = j < 0 ? 0 : j >= ncols ? ncols - 1 : j;
+                upd[u] = in(w) && i <= k2vec[jc];
+            } else {
+                upd[u] = in(w) && (mode == 0 ? notsrc : notsrc && zlim <= q.z[u]);
+            }
+        }
+        store_batch(pb - L);
+        load_batch(pb + L, nq);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int w = pb + u - t;
+            float *P = pub + (u & 1) * PB;
+            P[t] = prev;
+            P[NT + t] = next;
+            lds_barrier();
+            const float vb = P[rbehind], va = P[rahead];
+            float nw = cur;
+            if (upd[u]) {
+                // physical neighbours U[n-ncols], U[n+ncols], U[n-1], U[n+1] and the
+                // reference's edge rules (src/RT_pfsm2d.h:60-75)
+                const float um = s1 > 0 ? vb : va, up = s1 > 0 ? va : vb;
+                const float lm = s2 > 0 ? prev : next, lp = s2 > 0 ? next : prev;
+                const bool left = s2 > 0 ? w == 0 : w == ncols - 1;
+                const bool right = s2 > 0 ? w == ncols - 1 : w == 0;
+                const float b = top ? up : bottom ? um : (um < up ? um : up);
+                const float a = left ? lp : right ? lm : (lp < lm ? lp : lm);
+                const float qv = eik2d_f32(a, b, sl[u], fdx, fdz, fwx, fwz);
+                nw = cur < qv ? cur : qv;
+                changed |= nw != cur;
+            }
+            chg[u] = __float_as_uint(nw) != __float_as_uint(cur);
+            sval[u] = nw;
+            prev = nw;
+            cur = next;
+            next = q.u[u];
+        }
+    };
+    const int pend = nrows + ncols - 1;
+    int pl = 0;  // start of the last batch run
+    for (int pb = 0;; pb += 2 * L) {
+        batch(pb, qa, qb);
+        pl = pb;
+        if (pb + L >= pend) break;
+        batch(pb + L, qb, qa);
+        pl = pb + L;
+        if (pb + 2 * L >= pend) break;
+    }
+    wait_vm_all();
+    store_batch(pl);
+    return changed;
+}
+
+// Adjoint form of the row window: interior rows and columns only; (lambda, T) pairs
+// travel together (T read-only).  The boundary rows, never updated, are read from
+// memory by the first and last interior row.
+__device__ bool adj2d_window(float *__restrict__ U, const float *__restrict__ T,
+                             const unsigned char *__restrict__ tag, const Grid2 &g, int s1,
+                             int s2)
+{
+    constexpr int NT = W2NT, L = W2L, PB = 2 * NT + 1, INFSLOT = 2 * NT;
+    __shared__ float2 pub[2 * PB];
+    const int nrows = g.nrows, ncols = g.ncols, nri = nrows - 2, t = threadIdx.x;
+    const float2 NONE = make_float2(0.0f, PFAST_INF_F);
+    const bool valid = t < nri;
+    const int i = s1 > 0 ? 1 + t : nrows - 2 - t;
+    const long rb = valid ? (long)i * ncols : 0;
+    const bool hbehind = valid && t == 0, hahead = valid && t == nri - 1;
+    const long rbh = (long)(s1 > 0 ? i - 1 : i + 1) * ncols;  // rows behind / ahead
+    const long rah = (long)(s1 > 0 ? i + 1 : i - 1) * ncols;
+    const long rh = hbehind ? rbh : rah;                       // the halo row, if any
+    const bool halo = hbehind || hahead;
+    const int rbehind = valid && t > 0 ? t - 1 : INFSLOT;
+    const int rahead = t + 1 < nri ? NT + t + 1 : INFSLOT;
+    const unsigned N = (unsigned)nrows * ncols;
+    const rsrc_t RU = make_rsrc(U, 4u * N), RT = make_rsrc(T, 4u * N), RG = make_rsrc(tag, N);
+    auto col = [&](int w) { return s2 > 0 ? w : ncols - 1 - w; };
+    // a lane without a halo row reads past the arrays (0, no memory touched)
+    auto ld2 = [&](long r, int w, bool use) {
+        const unsigned e = use ? (unsigned)(r + col(w)) : 0x3FFFFFFFu;
+        return make_float2(bload(RU, 4u * e), bload(RT, 4u * e));
+    };
+    auto in = [&](int w) { return valid && w >= 1 && w <= ncols - 2; };
+    struct Q {
+        float2 u[L], h[L];
+        unsigned char tg[L];
+    };
+    auto load_batch = [&](int pb, Q &q) {
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int w = pb + u - t + 1;
+            q.u[u] = ld2(rb, w + 2, true);
+            q.tg[u] = (unsigned char)bload_u8(RG, (unsigned)(rb + col(w)));
+            q.h[u] = ld2(rh, w, halo);
+        }
+    };
+    float2 prev = ld2(rb, -t, true), cur = ld2(rb, 1 - t, true), next = ld2(rb, 2 - t, true);
+    Q qa, qb;
+    load_batch(0, qa);
+    if (t == 0) pub[INFSLOT] = pub[PB + INFSLOT] = NONE;
+    __syncthreads();
+    bool changed = false;
+    float sval[L];
+    bool chg[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) {
+        sval[u] = 0.0f;
+        chg[u] = false;
+    }
+    auto store_batch = [&](int pb) {
+#pragma unroll
+        for (int u = 0; u < L; u++)
+            if (chg[u]) bstore(RU, 4u * (unsigned)(rb + col(pb + u - t + 1)), sval[u]);
+    };
+    auto batch = [&](int pb, const Q &q, Q &nq) {
+        wait_vm_all();
+        store_batch(pb - L);
+        load_batch(pb + L, nq);
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            const int w = pb + u - t + 1;
+            float2 *P = pub + (u & 1) * PB;
+            P[t] = prev;
+            P[NT + t] = next;
+            lds_barrier();
+            const float2 vb = hbehind ? q.h[u] : P[rbehind];
+            const float2 va = hahead ? q.h[u] : P[rahead];
+            float nw = cur.x;
+            if (in(w) && !q.tg[u]) {
+                // physical neighbours n-ncols, n+ncols, n-1, n+1 (src/RT_padjoint2d.h)
+                const float2 rm = s1 > 0 ? vb : va, rp = s1 > 0 ? va : vb;
+                const float2 cm = s2 > 0 ? prev : next, cp = s2 > 0 ? next : prev;
+                nw = Fast32::adj(cur.y, cp.y, cm.y, rp.y, rm.y, cp.x, cm.x, rp.x, rm.x, g);
+                changed |= nw != cur.x;
+            }
+            chg[u] = __float_as_uint(nw) != __float_as_uint(cur.x);
+            sval[u] = nw;
+            prev = make_float2(nw, cur.y);
+            cur = next;
+            next = q.u[u];
+        }
+    };
+    const int pend = nri + ncols - 3;
+    int pl = 0;
+    for (int pb = 0;; pb += 2 * L) {
+        batch(pb, qa, qb);
+        pl = pb;
+        if (pb + L >= pend) break;
+        batch(pb + L, qb, qa);
+        pl = pb + L;
+        if (pb + 2 * L >= pend) break;
+    }
+    wait_vm_all();
+    store_batch(pl);
+    return changed;
+}
+
 // One 2D FSM solve (all cycles) by the calling work-group.  mode 0: all nodes but
 // the source; 1: also only (i+1)*dz <= Z depth of column j (down-going,
 // src/RT_pfsm2d.h:174); 2: only i <= k2vec[j] (up-going, :309).
@@ -191,6 +436,13 @@ __device__ int wg_fsm2d(const double *__restrict__ vp, typename P::real *__restr
         for (int o = 0; o < 4; o++) {
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
+            if constexpr (!P::exact) {
+                if (blockDim.x == W2NT && nrows <= W2NT) {
+                    if (fsm2d_window(vp, U, si, sj, mode, Zr, k2vec, g, s1, s2)) *s_flag = 1;
+                    __syncthreads();
+                    continue;
+                }
+            }
             for (int d = 0; d < nplanes; d++) {
                 const int ilo = d - (ncols - 1) > 0 ? d - (ncols - 1) : 0;
                 const int ihi = d < nrows - 1 ? d : nrows - 1;
@@ -258,8 +510,9 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
                                                    int *__restrict__ k2out)
 {
     using real = typename P::real;
-    extern __shared__ int s_k2[];  // ncols ints (+1 flag)
+    extern __shared__ int s_k2[];  // ncols ints, a flag, ncols ints
     int *s_flag = s_k2 + g.ncols;
+    int *s_kc = s_flag + 1;  // per column: last row above the reflector (candidate k2)
     const int b = blockIdx.x;
     const int nrows = g.nrows, ncols = g.ncols;
     const long N = (long)nrows * ncols;
@@ -284,22 +537,30 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
         real *Td = TTd + ((long)b * NumofR + ir) * N;
         for (long n = threadIdx.x; n < N; n += blockDim.x) Td[n] = U[n];
         __syncthreads();
-        // reflection points and seeds (src/RT_pfsm2d.h:195-281): sequential in the
-        // reference's column order because a seed can feed the next column's qx.
+        // reflection points and seeds (src/RT_pfsm2d.h:195-281).  A column's k2 (the
+        // last row kk passing the reference's test) depends on the reflector alone, so
+        // the columns are searched in parallel; the seeding pass stays sequential in
+        // the reference's column order because a seed can feed the next column's qx.
+        for (int jj = threadIdx.x; jj < ncols; jj += blockDim.x) {
+            const bool left = jj == 0, right = jj == ncols - 1;
+            int k2 = -1;
+            for (int kk = 0; kk < nrows - 1; kk++) {
+                bool cnd;
+                if (left) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj + 1) * 3];
+                else if (right) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj - 1) * 3];
+                else cnd = (kk + 1) * dz + dz <= Zr[jj * 3] &&
+                           (kk * dz + dz <= Zr[(jj - 1) * 3] || kk * dz + dz <= Zr[(jj + 1) * 3]);
+                if (cnd && kk >= k2) k2 = kk;
+            }
+            s_kc[jj] = k2;
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
             for (int step = 0; step < ncols && st == 0; step++) {
                 // columns 1..ncols-2, then 0, then ncols-1
                 const int jj = step < ncols - 2 ? step + 1 : (step == ncols - 2 ? 0 : ncols - 1);
                 const bool left = jj == 0, right = jj == ncols - 1;
-                int k2 = -1;
-                for (int kk = 0; kk < nrows - 1; kk++) {
-                    bool cnd;
-                    if (left) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj + 1) * 3];
-                    else if (right) cnd = (kk + 1) * dz + dz <= Zr[jj * 3] && kk * dz + dz <= Zr[(jj - 1) * 3];
-                    else cnd = (kk + 1) * dz + dz <= Zr[jj * 3] &&
-                               (kk * dz + dz <= Zr[(jj - 1) * 3] || kk * dz + dz <= Zr[(jj + 1) * 3]);
-                    if (cnd && kk >= k2) k2 = kk;
-                }
+                const int k2 = s_kc[jj];
                 if (Zr[jj * 3] <= nrows * dz && k2 > 0) {
                     const long r0 = (long)k2 * ncols;
                     double qx;
@@ -380,6 +641,13 @@ __device__ int wg_adj2d(typename P::real *__restrict__ U, typename P::real *__re
         for (int o = 0; o < 4; o++) {
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
+            if constexpr (!P::exact) {
+                if (blockDim.x == W2NT && nri >= 1 && nri <= W2NT && nci >= 1) {
+                    if (adj2d_window(U, T, tag, g, s1, s2)) *s_flag = 1;
+                    __syncthreads();
+                    continue;
+                }
+            }
             for (int d = 0; d < nplanes; d++) {
                 const int ilo = d - (nci - 1) > 0 ? d - (nci - 1) : 0;
                 const int ihi = d < nri - 1 ? d : nri - 1;
@@ -555,7 +823,7 @@ hipError_t launch_rt_fsm2d_f64(hipStream_t st, const double *vp, double *U, doub
                                int NumofR, int MAXSwp, Grid2 g, int *cycles, int *status,
                                int *k2out)
 {
-    size_t shm = sizeof(int) * (g.ncols + 1);
+    size_t shm = sizeof(int) * (2 * g.ncols + 1);
     hipLaunchKernelGGL(rt_fsm2d_wg<Ref64>, dim3(nb), dim3(256), shm, st, vp, U, TTd, TTu, p, Z,
                        src, NumofR, MAXSwp, g, cycles, status, k2out);
     return hipGetLastError();
@@ -565,7 +833,7 @@ hipError_t launch_rt_fsm2d_f32(hipStream_t st, const double *vp, float *U, float
                                float *TTu, const double *Z, const int *src, int nb, int NumofR,
                                int MAXSwp, Grid2 g, int *cycles, int *status, int *k2out)
 {
-    size_t shm = sizeof(int) * (g.ncols + 1);
+    size_t shm = sizeof(int) * (2 * g.ncols + 1);
     hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb), dim3(256), shm, st, vp, U, TTd, TTu,
                        (float *)nullptr, Z, src, NumofR, MAXSwp, g, cycles, status, k2out);
     return hipGetLastError();
