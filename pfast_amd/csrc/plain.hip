@@ -257,12 +257,18 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
             const int w = pb + u - t, j = col(w);
             sl[u] = (float)(1.0 / q.v[u]);
             const bool notsrc = !(i == si && j == sj);
+            bool up;
             if (mode == 2) {
                 const int jc = j < 0 ? 0 : j >= ncols ? ncols - 1 : j;
-                upd[u] = in(w) && i <= k2vec[jc];
+                up = in(w) && i <= k2vec[jc];
             } else {
-                upd[u] = in(w) && (mode == 0 ? notsrc : notsrc && zlim <= q.z[u]);
+                up = in(w) && (mode == 0 ? notsrc : notsrc && zlim <= q.z[u]);
             }
+            // materialised here: left to the compiler, the FP64 division and compare
+            // sink into each step's update and lengthen the step's dependent chain
+            int upi = up;
+            asm volatile("" : "+v"(sl[u]), "+v"(upi));
+            upd[u] = upi != 0;
         }
         store_batch(pb - L);
         load_batch(pb + L, nq);
@@ -514,6 +520,10 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
     int *s_flag = s_k2 + g.ncols;
     int *s_kc = s_flag + 1;  // per column: last row above the reflector (candidate k2)
     const int b = blockIdx.x;
+    // roles (grid y = 2, FP32 form with reflectors): block y 0 runs the reflection
+    // solves in the up-going field itself, block y 1 the transmission solve, which
+    // does not depend on them; with one block per source both run in turn in U
+    const int role = gridDim.y > 1 ? (int)blockIdx.y : -1;
     const int nrows = g.nrows, ncols = g.ncols;
     const long N = (long)nrows * ncols;
     real *U = Uall + b * N;
@@ -523,19 +533,21 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
     int st = 0;
     for (int j = threadIdx.x; j < ncols; j += blockDim.x) s_k2[j] = -1;
     __syncthreads();
-    for (int ir = 0; ir < NumofR && st == 0; ir++) {
+    for (int ir = 0; ir < NumofR && st == 0 && role != 1; ir++) {
+        // working field of this reflector's solves: U, or the up-going field itself
+        real *W = role == 0 ? TTu + ((long)b * NumofR + ir) * N : U;
         const double *Zr = Z + (long)ir * ncols * 3;
         for (long n = threadIdx.x; n < N; n += blockDim.x) {
-            U[n] = PFAST_INF_D;
+            W[n] = PFAST_INF_D;
             if (pbuf) pbuf[n] = 0;
         }
         __syncthreads();
-        if (threadIdx.x == 0) U[(long)si * ncols + sj] = 0;
+        if (threadIdx.x == 0) W[(long)si * ncols + sj] = 0;
         __syncthreads();
-        int c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 1, Zr, s_k2, MAXSwp, g, s_flag);
+        int c = wg_fsm2d<P>(vp, W, pbuf, si, sj, 1, Zr, s_k2, MAXSwp, g, s_flag);
         if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR) + 1 + 2 * ir] = c;
         real *Td = TTd + ((long)b * NumofR + ir) * N;
-        for (long n = threadIdx.x; n < N; n += blockDim.x) Td[n] = U[n];
+        for (long n = threadIdx.x; n < N; n += blockDim.x) Td[n] = W[n];
         __syncthreads();
         // reflection points and seeds (src/RT_pfsm2d.h:195-281).  A column's k2 (the
         // last row kk passing the reference's test) depends on the reflector alone, so
@@ -564,16 +576,16 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
                 if (Zr[jj * 3] <= nrows * dz && k2 > 0) {
                     const long r0 = (long)k2 * ncols;
                     double qx;
-                    if (left) qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj]) / dx;
-                    else if (right) qx = ((double)U[r0 + jj] - (double)U[r0 + jj - 1]) / dx;
+                    if (left) qx = ((double)W[r0 + jj + 1] - (double)W[r0 + jj]) / dx;
+                    else if (right) qx = ((double)W[r0 + jj] - (double)W[r0 + jj - 1]) / dx;
                     else if (k2 * dz + dz <= Zr[(jj - 1) * 3] && k2 * dz + dz <= Zr[(jj + 1) * 3])
-                        qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj - 1]) / 2.0 / dx;
+                        qx = ((double)W[r0 + jj + 1] - (double)W[r0 + jj - 1]) / 2.0 / dx;
                     else if (k2 * dz + dz <= Zr[(jj - 1) * 3])
-                        qx = ((double)U[r0 + jj] - (double)U[r0 + jj - 1]) / dx;
+                        qx = ((double)W[r0 + jj] - (double)W[r0 + jj - 1]) / dx;
                     else
-                        qx = ((double)U[r0 + jj + 1] - (double)U[r0 + jj]) / dx;
-                    const double v = reflect_seed_dev<P>(U, Zr, jj, k2, qx, dz, ncols);
-                    U[(long)(k2 + 1) * ncols + jj] = (real)v;
+                        qx = ((double)W[r0 + jj + 1] - (double)W[r0 + jj]) / dx;
+                    const double v = reflect_seed_dev<P>(W, Zr, jj, k2, qx, dz, ncols);
+                    W[(long)(k2 + 1) * ncols + jj] = (real)v;
                     s_k2[jj] = k2;
                     if (v < 0) st = 2;
                 } else if (Zr[jj * 3] > nrows * dz) {
@@ -591,17 +603,18 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
         // reset above the reflection points (src/RT_pfsm2d.h:289-295)
         for (long n = threadIdx.x; n < N; n += blockDim.x) {
             const int i = (int)(n / ncols), j = (int)(n % ncols);
-            if (i <= s_k2[j]) U[n] = PFAST_INF_D;
+            if (i <= s_k2[j]) W[n] = PFAST_INF_D;
             if (pbuf) pbuf[n] = 0;
         }
         __syncthreads();
-        c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 2, Zr, s_k2, MAXSwp, g, s_flag);
+        c = wg_fsm2d<P>(vp, W, pbuf, si, sj, 2, Zr, s_k2, MAXSwp, g, s_flag);
         if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR) + 2 + 2 * ir] = c;
         real *Tu = TTu + ((long)b * NumofR + ir) * N;
-        for (long n = threadIdx.x; n < N; n += blockDim.x) Tu[n] = U[n];
+        if (W != Tu)
+            for (long n = threadIdx.x; n < N; n += blockDim.x) Tu[n] = W[n];
         __syncthreads();
     }
-    if (st == 0) {
+    if (st == 0 && role != 0) {
         for (long n = threadIdx.x; n < N; n += blockDim.x) {
             U[n] = PFAST_INF_D;
             if (pbuf) pbuf[n] = 0;
@@ -612,6 +625,7 @@ __global__ __launch_bounds__(256) void rt_fsm2d_wg(const double *__restrict__ vp
         int c = wg_fsm2d<P>(vp, U, pbuf, si, sj, 0, Z, s_k2, MAXSwp, g, s_flag);
         if (threadIdx.x == 0 && cycles) cycles[b * (1 + 2 * NumofR)] = c;
     }
+    if (role == 1) return;  // the reflection block reports k2 and the status
     if (k2out)
         for (int j = threadIdx.x; j < ncols; j += blockDim.x) k2out[(long)b * ncols + j] = s_k2[j];
     if (threadIdx.x == 0 && status) status[b] = st;
@@ -676,51 +690,118 @@ __device__ int wg_adj2d(typename P::real *__restrict__ U, typename P::real *__re
     return swept;
 }
 
-// Receiver BC for one source (thread 0 of the block), src/RT_padjoint2d.h:298-330
+// Receiver BC for one source (all threads of the block), src/RT_padjoint2d.h:298-330
 // (transmission, checkSrc=1, pick T_obs<100) and :402-435 (reflection, checkSrc=0,
-// pick TR_obs != 0).  Adds to the running EE in receiver order, as the reference
-// does; sets *first_nc to !isconv(0,U).
+// pick TR_obs != 0).  A receiver's boundary value and misfit term read only T, so the
+// threads compute them for 256 receivers at a time; thread 0 then applies them in
+// receiver order -- a node picked twice keeps the last value and EE (thread 0's) adds
+// the terms in the reference's order.  Sets *s_nc to !isconv(0,U).
 template <class P>
 __device__ void bc2d(typename P::real *U, unsigned char *tag, const typename P::real *T,
                      const double *obs, int obs_stride, int obs_off, bool transmission,
                      double Cr, const int *rec, const double *nxy, long ch0, int nrec, int si,
-                     int sj, const Grid2 &g, int *first_nc, double &EE)
+                     int sj, const Grid2 &g, int *s_nc, double &EE)
 {
-    const int nrows = g.nrows, ncols = g.ncols;
-    for (int idx = 0; idx < nrec; idx++) {
-        const long r = idx + ch0;
-        const double o = obs[r * obs_stride + obs_off];
-        if (transmission ? !(o < 100) : !(o != 0)) continue;
-        const int i = rec[r * 2 + 0], j = rec[r * 2 + 1];
-        const long n = (long)i * ncols + j;
-        tag[n] = 1;
-        const double Tn = (double)T[n];
-        EE += Cr * (o - Tn) * (o - Tn) / 2.0;
-        if (transmission && i == si && j == sj) continue;
-        const double nny = nxy[r * 2 + 0], nnx = nxy[r * 2 + 1];
-        double Tx, Tz;
-        if (i + 1 >= nrows) Tz = (Tn - (double)T[n - ncols]) / g.dz;
-        else Tz = ((double)T[n + ncols] - Tn) / g.dz;
-        if (j + 1 >= ncols) Tx = (Tn - (double)T[n - 1]) / g.dx;
-        else Tx = ((double)T[n + 1] - Tn) / g.dx;
-        if (fabs(o - Tn) > 1e-9 && fabs(nnx * Tx + nny * Tz) > 1e-6)
-            U[n] = (typename P::real)(Cr * (o - Tn) / (nnx * Tx + nny * Tz));
-        else
-            U[n] = 0;
+    using real = typename P::real;
+    constexpr int NB = 256;  // receivers per round (>= blockDim.x)
+    __shared__ double s_term[NB];
+    __shared__ real s_lam[NB];
+    __shared__ int s_node[NB], s_kind[NB];  // kind 0: not picked, 1: picked, 2: with lambda
+    const int nrows = g.nrows, ncols = g.ncols, tid = threadIdx.x;
+    auto picked = [&](double o) { return transmission ? o < 100 : o != 0; };
+    for (int base = 0; base < nrec; base += (int)blockDim.x) {
+        const int idx = base + tid;
+        if (idx < nrec) {
+            const long r = idx + ch0;
+            const double o = obs[r * obs_stride + obs_off];
+            int kind = 0;
+            long n = 0;
+            double term = 0.0;
+            real lam = 0;
+            if (picked(o)) {
+                const int i = rec[r * 2 + 0], j = rec[r * 2 + 1];
+                n = (long)i * ncols + j;
+                const double Tn = (double)T[n];
+                term = Cr * (o - Tn) * (o - Tn) / 2.0;
+                kind = 1;
+                if (!(transmission && i == si && j == sj)) {
+                    const double nny = nxy[r * 2 + 0], nnx = nxy[r * 2 + 1];
+                    double Tx, Tz;
+                    if (i + 1 >= nrows) Tz = (Tn - (double)T[n - ncols]) / g.dz;
+                    else Tz = ((double)T[n + ncols] - Tn) / g.dz;
+                    if (j + 1 >= ncols) Tx = (Tn - (double)T[n - 1]) / g.dx;
+                    else Tx = ((double)T[n + 1] - Tn) / g.dx;
+                    if (fabs(o - Tn) > 1e-9 && fabs(nnx * Tx + nny * Tz) > 1e-6)
+                        lam = (real)(Cr * (o - Tn) / (nnx * Tx + nny * Tz));
+                    else
+                        lam = 0;
+                    kind = 2;
+                }
+            }
+            s_term[tid] = term;
+            s_lam[tid] = lam;
+            s_node[tid] = (int)n;
+            s_kind[tid] = kind;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int m = nrec - base < (int)blockDim.x ? nrec - base : (int)blockDim.x;
+            for (int k = 0; k < m; k++) {
+                if (!s_kind[k]) continue;
+                const long n = s_node[k];
+                tag[n] = 1;
+                EE += s_term[k];
+                if (s_kind[k] == 2) U[n] = s_lam[k];
+            }
+        }
+        __syncthreads();
     }
-    int nc = 0;
-    for (int idx = 0; idx < nrec && !nc; idx++) {
+    // some picked receiver's lambda above 1e-9 in magnitude (after all the writes)
+    if (tid == 0) *s_nc = 0;
+    __syncthreads();
+    for (int idx = tid; idx < nrec; idx += (int)blockDim.x) {
         const long r = idx + ch0;
-        const double o = obs[r * obs_stride + obs_off];
-        if (transmission ? !(o < 100) : !(o != 0)) continue;
+        if (!picked(obs[r * obs_stride + obs_off])) continue;
         const long n = (long)rec[r * 2 + 0] * ncols + rec[r * 2 + 1];
-        if (fabs((double)U[n]) > 1e-9) nc = 1;
+        if (fabs((double)U[n]) > 1e-9) *s_nc = 1;
     }
-    *first_nc = nc;
+    __syncthreads();
 }
 
 // Tadjoint_fsm2d (src/RT_padjoint2d.h:288-380, Cr-weighted overload :67-157) for
-// source `b = blockIdx.x` of a batch: U [b][N], tag scratch [b][N].
+// source b of a batch, by the calling block: U [b][N], tag scratch [b][N].
+template <class P>
+__device__ void tadj2d_src(typename P::real *__restrict__ Uall, typename P::real *__restrict__ pall,
+                           unsigned char *__restrict__ tagall,
+                           const typename P::real *__restrict__ Tall,
+                           const double *__restrict__ T_obs, double Cr,
+                           const int *__restrict__ rec, const double *__restrict__ nxy,
+                           const long *__restrict__ ch0, const int *__restrict__ srnum,
+                           const int *__restrict__ src, int MAXSwp, const Grid2 &g,
+                           double *__restrict__ EE, int *__restrict__ cycles, int b)
+{
+    __shared__ int s_flag[2];
+    const long N = (long)g.nrows * g.ncols;
+    auto *U = Uall + b * N;
+    auto *pbuf = pall ? pall + b * N : nullptr;
+    auto *tag = tagall + b * N;
+    const auto *T = Tall + b * N;
+    for (long n = threadIdx.x; n < N; n += blockDim.x) {
+        U[n] = 0;
+        tag[n] = 0;
+        if (pbuf) pbuf[n] = 0;
+    }
+    __syncthreads();
+    double e = 0.0;
+    bc2d<P>(U, tag, T, T_obs, 1, 0, true, Cr, rec, nxy, ch0[b], srnum[b], src[b * 2],
+            src[b * 2 + 1], g, &s_flag[1], e);
+    if (threadIdx.x == 0) EE[b] = e;
+    int count = 0;
+    const int nc = s_flag[1];
+    int sw = wg_adj2d<P>(U, pbuf, T, tag, count, nc, MAXSwp, g, s_flag);
+    if (threadIdx.x == 0 && cycles) cycles[b] = sw;
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void tadj2d_wg(typename P::real *__restrict__ Uall,
                                                  typename P::real *__restrict__ pall,
@@ -734,53 +815,23 @@ __global__ __launch_bounds__(256) void tadj2d_wg(typename P::real *__restrict__ 
                                                  const int *__restrict__ src, int MAXSwp, Grid2 g,
                                                  double *__restrict__ EE, int *__restrict__ cycles)
 {
-    __shared__ int s_flag[2];
-    const int b = blockIdx.x;
-    const long N = (long)g.nrows * g.ncols;
-    auto *U = Uall + b * N;
-    auto *pbuf = pall ? pall + b * N : nullptr;
-    auto *tag = tagall + b * N;
-    const auto *T = Tall + b * N;
-    for (long n = threadIdx.x; n < N; n += blockDim.x) {
-        U[n] = 0;
-        tag[n] = 0;
-        if (pbuf) pbuf[n] = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int nc = 0;
-        double e = 0.0;
-        bc2d<P>(U, tag, T, T_obs, 1, 0, true, Cr, rec, nxy, ch0[b], srnum[b], src[b * 2],
-                src[b * 2 + 1], g, &nc, e);
-        EE[b] = e;
-        s_flag[1] = nc;
-    }
-    __syncthreads();
-    int count = 0;
-    const int nc = s_flag[1];
-    int sw = wg_adj2d<P>(U, pbuf, T, tag, count, nc, MAXSwp, g, s_flag);
-    if (threadIdx.x == 0 && cycles) cycles[b] = sw;
+    tadj2d_src<P>(Uall, pall, tagall, Tall, T_obs, Cr, rec, nxy, ch0, srnum, src, MAXSwp, g, EE,
+                  cycles, blockIdx.x);
 }
 
-// Radjoint_fsm2d (src/RT_padjoint2d.h:382-473, Cr-weighted overload :159-285).
-// Tu [b][NumofR][N]; UR [b][N]; U/p/tag scratch [b][N].
+// Radjoint_fsm2d (src/RT_padjoint2d.h:382-473, Cr-weighted overload :159-285) for
+// source b, by the calling block.  Tu [b][NumofR][N]; UR [b][N]; U/p/tag scratch [b][N].
 template <class P>
-__global__ __launch_bounds__(256) void radj2d_wg(typename P::real *__restrict__ URall,
-                                                 typename P::real *__restrict__ Uall,
-                                                 typename P::real *__restrict__ pall,
-                                                 unsigned char *__restrict__ tagall,
-                                                 const typename P::real *__restrict__ Tuall,
-                                                 const double *__restrict__ TR_obs, double Cr,
-                                                 const int *__restrict__ rec,
-                                                 const double *__restrict__ nxy,
-                                                 const long *__restrict__ ch0,
-                                                 const int *__restrict__ srnum,
-                                                 const int *__restrict__ src, int NumofR,
-                                                 int MAXSwp, Grid2 g, double *__restrict__ EE,
-                                                 int *__restrict__ cycles)
+__device__ void radj2d_src(typename P::real *__restrict__ URall, typename P::real *__restrict__ Uall,
+                           typename P::real *__restrict__ pall, unsigned char *__restrict__ tagall,
+                           const typename P::real *__restrict__ Tuall,
+                           const double *__restrict__ TR_obs, double Cr,
+                           const int *__restrict__ rec, const double *__restrict__ nxy,
+                           const long *__restrict__ ch0, const int *__restrict__ srnum,
+                           const int *__restrict__ src, int NumofR, int MAXSwp, const Grid2 &g,
+                           double *__restrict__ EE, int *__restrict__ cycles, int b)
 {
     __shared__ int s_flag[2];
-    const int b = blockIdx.x;
     const long N = (long)g.nrows * g.ncols;
     auto *UR = URall + b * N;
     auto *U = Uall + b * N;
@@ -799,13 +850,8 @@ __global__ __launch_bounds__(256) void radj2d_wg(typename P::real *__restrict__ 
             if (pbuf) pbuf[n] = 0;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int nc = 0;
-            bc2d<P>(U, tag, T, TR_obs, NumofR, ir, false, Cr, rec, nxy, ch0[b], srnum[b],
-                    src[b * 2], src[b * 2 + 1], g, &nc, EEacc);
-            s_flag[1] = nc;
-        }
-        __syncthreads();
+        bc2d<P>(U, tag, T, TR_obs, NumofR, ir, false, Cr, rec, nxy, ch0[b], srnum[b],
+                src[b * 2], src[b * 2 + 1], g, &s_flag[1], EEacc);
         const int nc = s_flag[1];
         int sw = wg_adj2d<P>(U, pbuf, T, tag, count, nc, MAXSwp, g, s_flag);
         if (threadIdx.x == 0 && cycles) cycles[b * NumofR + ir] = sw;
@@ -813,6 +859,45 @@ __global__ __launch_bounds__(256) void radj2d_wg(typename P::real *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) EE[b] = EEacc;
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void radj2d_wg(typename P::real *__restrict__ URall,
+                                                 typename P::real *__restrict__ Uall,
+                                                 typename P::real *__restrict__ pall,
+                                                 unsigned char *__restrict__ tagall,
+                                                 const typename P::real *__restrict__ Tuall,
+                                                 const double *__restrict__ TR_obs, double Cr,
+                                                 const int *__restrict__ rec,
+                                                 const double *__restrict__ nxy,
+                                                 const long *__restrict__ ch0,
+                                                 const int *__restrict__ srnum,
+                                                 const int *__restrict__ src, int NumofR,
+                                                 int MAXSwp, Grid2 g, double *__restrict__ EE,
+                                                 int *__restrict__ cycles)
+{
+    radj2d_src<P>(URall, Uall, pall, tagall, Tuall, TR_obs, Cr, rec, nxy, ch0, srnum, src, NumofR,
+                  MAXSwp, g, EE, cycles, blockIdx.x);
+}
+
+// Both adjoints of a batch in one launch (FP32 form): block y 0 runs source x's
+// transmission adjoint, block y 1 its reflection adjoints; they share no buffer (the
+// reflection adjoint has its own tag scratch).
+__global__ __launch_bounds__(256) void adj2d_pair_wg(
+    float *__restrict__ LT, unsigned char *__restrict__ tagT, const float *__restrict__ T,
+    const double *__restrict__ T_obs, float *__restrict__ LR, float *__restrict__ Uw,
+    unsigned char *__restrict__ tagR, const float *__restrict__ Tu,
+    const double *__restrict__ TR_obs, double Cr, const int *__restrict__ rec,
+    const double *__restrict__ nxy, const long *__restrict__ ch0, const int *__restrict__ srnum,
+    const int *__restrict__ src, int NumofR, int MAXSwp, Grid2 g, double *__restrict__ EEt,
+    int *__restrict__ tcyc, double *__restrict__ EEr, int *__restrict__ rcyc)
+{
+    if (blockIdx.y == 0)
+        tadj2d_src<Fast32>(LT, nullptr, tagT, T, T_obs, Cr, rec, nxy, ch0, srnum, src, MAXSwp, g,
+                           EEt, tcyc, blockIdx.x);
+    else
+        radj2d_src<Fast32>(LR, Uw, nullptr, tagR, Tu, TR_obs, Cr, rec, nxy, ch0, srnum, src,
+                           NumofR, MAXSwp, g, EEr, rcyc, blockIdx.x);
 }
 
 // ------------------------------------------------------------ launch wrappers
@@ -834,7 +919,8 @@ hipError_t launch_rt_fsm2d_f32(hipStream_t st, const double *vp, float *U, float
                                int MAXSwp, Grid2 g, int *cycles, int *status, int *k2out)
 {
     size_t shm = sizeof(int) * (2 * g.ncols + 1);
-    hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb), dim3(256), shm, st, vp, U, TTd, TTu,
+    hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb, NumofR > 0 ? 2 : 1), dim3(256), shm, st, vp,
+                       U, TTd, TTu,
                        (float *)nullptr, Z, src, NumofR, MAXSwp, g, cycles, status, k2out);
     return hipGetLastError();
 }
@@ -880,6 +966,19 @@ hipError_t launch_radj2d_f32(hipStream_t st, float *UR, float *U, unsigned char 
     hipLaunchKernelGGL(radj2d_wg<Fast32>, dim3(nb), dim3(256), 0, st, UR, U, (float *)nullptr,
                        tag, Tu, TR_obs, Cr, rec, nxy, ch0, srnum, src, NumofR, MAXSwp, g, EE,
                        cycles);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj2d_pair_f32(hipStream_t st, float *LT, unsigned char *tagT, const float *T,
+                                 const double *T_obs, float *LR, float *Uw, unsigned char *tagR,
+                                 const float *Tu, const double *TR_obs, double Cr, const int *rec,
+                                 const double *nxy, const long *ch0, const int *srnum,
+                                 const int *src, int nb, int NumofR, int MAXSwp, Grid2 g,
+                                 double *EEt, int *tcyc, double *EEr, int *rcyc)
+{
+    hipLaunchKernelGGL(adj2d_pair_wg, dim3(nb, 2), dim3(256), 0, st, LT, tagT, T, T_obs, LR, Uw,
+                       tagR, Tu, TR_obs, Cr, rec, nxy, ch0, srnum, src, NumofR, MAXSwp, g, EEt,
+                       tcyc, EEr, rcyc);
     return hipGetLastError();
 }
 

@@ -34,6 +34,13 @@ hipError_t launch_radj2d_f32(hipStream_t st, float *UR, float *U, unsigned char 
                              const double *nxy, const long *ch0, const int *srnum,
                              const int *src, int nb, int NumofR, int MAXSwp, Grid2 g,
                              double *EE, int *cycles);
+// Both FP32 adjoints of a batch in one launch (separate tag scratch for each).
+hipError_t launch_adj2d_pair_f32(hipStream_t st, float *LT, unsigned char *tagT, const float *T,
+                                 const double *T_obs, float *LR, float *Uw, unsigned char *tagR,
+                                 const float *Tu, const double *TR_obs, double Cr, const int *rec,
+                                 const double *nxy, const long *ch0, const int *srnum,
+                                 const int *src, int nb, int NumofR, int MAXSwp, Grid2 g,
+                                 double *EEt, int *tcyc, double *EEr, int *rcyc);
 hipError_t launch_init3d_f64(hipStream_t st, double *U, long N, long srcnode);
 hipError_t launch_fsm3d_ordering_f64(hipStream_t st, const double *vp, double *U, double *p,
                                      int *flag, int si, int sj, int sk, int s1, int s2, int s3,

@@ -120,15 +120,18 @@ int issue_adj2d(pfast_shots2d *s, int MAXSwp)
     const Grid2 g = grid_of(m);
     const int R = m->NumofR;
     time_begin(ctx, 2);
-    HIPCHK(launch_tadj2d_f32(st, s->LT, s->tag, s->TT, s->TT_obs, 1.0, s->rec, s->nxy, s->dch0,
-                             s->srnum, s->src, s->nsrc, MAXSwp, g, s->EEt, s->tcyc));
-    time_end(ctx);
     if (R > 0) {
-        time_begin(ctx, 2);
-        HIPCHK(launch_radj2d_f32(st, s->LR, s->Uw, s->tag, s->TTu, s->TR_obs, 1.0, s->rec, s->nxy,
-                                 s->dch0, s->srnum, s->src, s->nsrc, R, MAXSwp, g, s->EEr, s->rcyc));
+        // the two adjoints of each source run side by side (tag scratch [2][nsrc][N])
+        const size_t F = (size_t)s->nsrc * m->nrows * m->ncols;
+        HIPCHK(launch_adj2d_pair_f32(st, s->LT, s->tag, s->TT, s->TT_obs, s->LR, s->Uw,
+                                     s->tag + F, s->TTu, s->TR_obs, 1.0, s->rec, s->nxy, s->dch0,
+                                     s->srnum, s->src, s->nsrc, R, MAXSwp, g, s->EEt, s->tcyc,
+                                     s->EEr, s->rcyc));
         time_end(ctx);
     } else {
+        HIPCHK(launch_tadj2d_f32(st, s->LT, s->tag, s->TT, s->TT_obs, 1.0, s->rec, s->nxy, s->dch0,
+                                 s->srnum, s->src, s->nsrc, MAXSwp, g, s->EEt, s->tcyc));
+        time_end(ctx);
         HIPCHK(hipMemsetAsync(s->EEr, 0, s->nsrc * sizeof(double), st));
     }
     return PFAST_OK;
@@ -150,7 +153,7 @@ size_t footprint2d(int nrows, int ncols, int NumofR, int nsrc, long nrec, bool a
     const size_t N = (size_t)nrows * ncols, F = (size_t)nsrc * N, R1 = NumofR ? NumofR : 1;
     size_t b = F * 4 * (1 + 2 * R1) + N * 8 + (size_t)nsrc * (4 * (4 + 2 * NumofR + ncols) + 8);
     if (adjoint)
-        b += F * (4 * 3 + 1) + N * 8 * 4 + (size_t)nrec * (8 + 16 + 8 + 8 * NumofR) +
+        b += F * (4 * 3 + 2) + N * 8 * 4 + (size_t)nrec * (8 + 16 + 8 + 8 * NumofR) +
              (size_t)nsrc * (8 * 2 + 4 * (1 + R1));
     return b;
 }
@@ -288,7 +291,7 @@ int pfast_shots2d_create(pfast_model2d *m, int nsrc, const int *src, const int *
               ok(pfast::dupload(m->ctx, &s->TT_obs, TT_obs, s->nrec)) &&
               ok(pfast::dupload(m->ctx, &s->TR_obs, TR_obs, (size_t)s->nrec * R)) &&
               ok(pfast::dalloc(m->ctx, &s->LT, F)) && ok(pfast::dalloc(m->ctx, &s->LR, F)) &&
-              ok(pfast::dalloc(m->ctx, &s->Uw, F)) && ok(pfast::dalloc(m->ctx, &s->tag, F)) &&
+              ok(pfast::dalloc(m->ctx, &s->Uw, F)) && ok(pfast::dalloc(m->ctx, &s->tag, 2 * F)) &&
               ok(pfast::dalloc(m->ctx, &s->EEt, nsrc)) && ok(pfast::dalloc(m->ctx, &s->EEr, nsrc)) &&
               ok(pfast::dalloc(m->ctx, &s->tcyc, nsrc)) && ok(pfast::dalloc(m->ctx, &s->rcyc, (size_t)nsrc * (R ? R : 1))) &&
               ok(pfast::dalloc(m->ctx, &s->gT, N)) && ok(pfast::dalloc(m->ctx, &s->gR, N)) &&
