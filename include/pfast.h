@@ -167,6 +167,12 @@ int pfast_ctx_timing_reset(pfast_ctx *ctx);
  * in tile shape and batch depth.  pfast_sweep_name: a variant's description, NULL
  * past the last one. */
 int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant);
+/* Batched 2D FP32 sweep form of this context (no counterpart in the reference, whose
+ * sweeps have one loop order): 0 (default) = row window, each work-group walking the
+ * rows of one source, with a source's independent solves (transmission / reflection,
+ * and their adjoints) in work-groups of their own; 1 = one work-group per source
+ * stepping the anti-diagonal planes.  Both give bit-identical results. */
+int pfast_ctx_set_sweep2d(pfast_ctx *ctx, int variant);
 const char *pfast_sweep_name(int variant);
 /* Streams (1..4) over which a batched 3D cycle splits its active source chunks, so
  * that the launches of one stream fill the idle tail of another's; results are

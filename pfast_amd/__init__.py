@@ -165,6 +165,12 @@ class Context:
         """Select the batched sweep kernel variant (0 = default); see sweep_variants()."""
         check(lib().pfast_ctx_set_sweep(self.h, int(variant)))
 
+    def set_sweep2d(self, variant):
+        """Batched 2D sweep form: 0 (default) row window with a source's independent
+        solves side by side, 1 one work-group per source stepping anti-diagonal planes
+        (bit-identical results)."""
+        check(lib().pfast_ctx_set_sweep2d(self.h, int(variant)))
+
     def set_tile_skip(self, enable=True):
         """Exact tile skipping of the batched 3D sweeps (default on; same results)."""
         check(lib().pfast_ctx_set_tile_skip(self.h, 1 if enable else 0))

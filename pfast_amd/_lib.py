@@ -54,6 +54,7 @@ _SIGS = {
     "pfast_ctx_timing_enable": [vpp, i],
     "pfast_ctx_timing_reset": [vpp],
     "pfast_ctx_set_sweep": [vpp, i],
+    "pfast_ctx_set_sweep2d": [vpp, i],
     "pfast_ctx_set_streams": [vpp, i],
     "pfast_ctx_set_tile_skip": [vpp, i],
     "pfast_model3d_create": [vpp, dp, i, i, i, d, d, d, C.POINTER(vpp)],

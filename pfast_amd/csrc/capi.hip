@@ -188,6 +188,14 @@ int pfast_ctx_set_sweep(pfast_ctx *ctx, int variant)
     return PFAST_OK;
 }
 
+int pfast_ctx_set_sweep2d(pfast_ctx *ctx, int variant)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ARGCHK(variant == 0 || variant == 1, "unknown 2D sweep variant");
+    ctx->sweep2d = variant;
+    return PFAST_OK;
+}
+
 const char *pfast_sweep_name(int variant)
 {
     if (variant < 0 || variant >= pfast::sweep_variant_count()) return nullptr;

@@ -29,6 +29,7 @@ struct pfast_ctx {
     long launches[3] = {0, 0, 0};
     double units[3] = {0, 0, 0};  // node-sweeps of real sources issued (batched paths)
     int sweep = 0;                // batched sweep kernel variant
+    int sweep2d = 0;              // batched 2D sweep form (0 row window, 1 plane loop)
     int nmodels = 0;              // live models bound to this context
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, int>> pending;  // (kind, event index of start)

@@ -443,7 +443,7 @@ __device__ int wg_fsm2d(const double *__restrict__ vp, typename P::real *__restr
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
             if constexpr (!P::exact) {
-                if (blockDim.x == W2NT && nrows <= W2NT) {
+                if (g.window && blockDim.x == W2NT && nrows <= W2NT) {
                     if (fsm2d_window(vp, U, si, sj, mode, Zr, k2vec, g, s1, s2)) *s_flag = 1;
                     __syncthreads();
                     continue;
@@ -656,7 +656,7 @@ __device__ int wg_adj2d(typename P::real *__restrict__ U, typename P::real *__re
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
             if constexpr (!P::exact) {
-                if (blockDim.x == W2NT && nri >= 1 && nri <= W2NT && nci >= 1) {
+                if (g.window && blockDim.x == W2NT && nri >= 1 && nri <= W2NT && nci >= 1) {
                     if (adj2d_window(U, T, tag, g, s1, s2)) *s_flag = 1;
                     __syncthreads();
                     continue;
@@ -919,7 +919,8 @@ hipError_t launch_rt_fsm2d_f32(hipStream_t st, const double *vp, float *U, float
                                int MAXSwp, Grid2 g, int *cycles, int *status, int *k2out)
 {
     size_t shm = sizeof(int) * (2 * g.ncols + 1);
-    hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb, NumofR > 0 ? 2 : 1), dim3(256), shm, st, vp,
+    hipLaunchKernelGGL(rt_fsm2d_wg<Fast32>, dim3(nb, NumofR > 0 && g.window ? 2 : 1), dim3(256),
+                       shm, st, vp,
                        U, TTd, TTu,
                        (float *)nullptr, Z, src, NumofR, MAXSwp, g, cycles, status, k2out);
     return hipGetLastError();

@@ -5,6 +5,9 @@
 struct Grid2 {
     int nrows, ncols;
     double dx, dz, wx, wz;  // spacings (x = columns, z = rows) and 1/h^2
+    // FP32 batched form: 1 = row-window sweeps and independent solves of a source in
+    // blocks of their own (default), 0 = plane loop, one block per source (same results)
+    int window = 1;
 };
 
 namespace pfast {

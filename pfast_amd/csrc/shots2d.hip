@@ -75,6 +75,7 @@ Grid2 grid_of(const pfast_model2d *m)
     g.dz = m->dz;
     g.wx = 1.0 / (m->dx * m->dx);
     g.wz = 1.0 / (m->dz * m->dz);
+    g.window = m->ctx->sweep2d == 0;
     return g;
 }
 
@@ -120,7 +121,7 @@ int issue_adj2d(pfast_shots2d *s, int MAXSwp)
     const Grid2 g = grid_of(m);
     const int R = m->NumofR;
     time_begin(ctx, 2);
-    if (R > 0) {
+    if (R > 0 && g.window) {
         // the two adjoints of each source run side by side (tag scratch [2][nsrc][N])
         const size_t F = (size_t)s->nsrc * m->nrows * m->ncols;
         HIPCHK(launch_adj2d_pair_f32(st, s->LT, s->tag, s->TT, s->TT_obs, s->LR, s->Uw,
@@ -131,8 +132,12 @@ int issue_adj2d(pfast_shots2d *s, int MAXSwp)
     } else {
         HIPCHK(launch_tadj2d_f32(st, s->LT, s->tag, s->TT, s->TT_obs, 1.0, s->rec, s->nxy, s->dch0,
                                  s->srnum, s->src, s->nsrc, MAXSwp, g, s->EEt, s->tcyc));
+        if (R > 0)
+            HIPCHK(launch_radj2d_f32(st, s->LR, s->Uw, s->tag, s->TTu, s->TR_obs, 1.0, s->rec,
+                                     s->nxy, s->dch0, s->srnum, s->src, s->nsrc, R, MAXSwp, g,
+                                     s->EEr, s->rcyc));
         time_end(ctx);
-        HIPCHK(hipMemsetAsync(s->EEr, 0, s->nsrc * sizeof(double), st));
+        if (R == 0) HIPCHK(hipMemsetAsync(s->EEr, 0, s->nsrc * sizeof(double), st));
     }
     return PFAST_OK;
 }

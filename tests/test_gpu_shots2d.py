@@ -136,3 +136,48 @@ def test_shots2d_fatal_exit_is_reported(ctx):
         sh.run(adjoint=False)
     sh.close()
     m.close()
+
+
+def _all_results(sh, nsrc, R):
+    out = {"info": np.asarray(sh.info()[0]), "k2": np.asarray(sh.info()[1])}
+    gT, gR, gRT, E = sh.gradient(0.5)
+    out.update(gT=gT, gR=gR, gRT=gRT, E=np.array(E))
+    for s in range(nsrc):
+        out[f"T{s}"] = sh.field(s, "T")
+        out[f"lT{s}"] = sh.field(s, "lambdaT")
+        for r in range(R):
+            out[f"Td{s}_{r}"] = sh.field(s, "TTd", r)
+            out[f"Tu{s}_{r}"] = sh.field(s, "TTu", r)
+        if R:
+            out[f"lR{s}"] = sh.field(s, "lambdaR")
+    return out
+
+
+@pytest.mark.parametrize("seed,NumofR,shape,nsrc", [
+    (71, 0, (23, 35), 5),
+    (72, 1, (29, 44), 6),
+    (73, 2, (31, 47), 7),
+])
+def test_row_window_matches_plane_loop_bitwise(ctx, seed, NumofR, shape, nsrc):
+    """The row-window sweeps (a work-group walking the rows, independent solves of a
+    source side by side) and the plane loop (one work-group per source stepping the
+    anti-diagonals) update every node with the same operands in the same order: every
+    field, reflection row, cycle count, gradient and misfit agrees bit for bit."""
+    import pfast_amd
+    c = cases.case2d(seed, shape=shape, NumofR=max(NumofR, 1), nsrc=nsrc, nrec_per=9)
+    Z = c["Z"][:NumofR] if NumofR else None
+    TR = c["TR_obs"][:, :max(NumofR, 1)]
+    res = []
+    for form in (0, 1):
+        ctx.set_sweep2d(form)
+        m = pfast_amd.Model2D(ctx, c["vp"], c["dx"], c["dz"], Z)
+        sh = pfast_amd.Shots2D(m, c["src"], c["srnum"], c["rec"], c["nxy"], c["T_obs"], TR)
+        sh.run(adjoint=True, MAXSwp=500)
+        res.append(_all_results(sh, nsrc, NumofR))
+        sh.close()
+        m.close()
+    ctx.set_sweep2d(0)
+    a, b = res
+    assert a.keys() == b.keys()
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]).view(np.uint8), np.asarray(b[k]).view(np.uint8)), k
