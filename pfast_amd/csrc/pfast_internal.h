@@ -59,10 +59,6 @@ __device__ __forceinline__ double bload_d(rsrc_t r, unsigned off)
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
-__device__ __forceinline__ int bload_i(rsrc_t r, unsigned off)
-{
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
-}
 __device__ __forceinline__ unsigned bload_u8(rsrc_t r, unsigned off)
 {
     return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
