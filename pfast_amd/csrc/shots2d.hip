@@ -172,6 +172,9 @@ int pfast_model2d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
 {
     ARGCHK(ctx && vp && out, "NULL argument");
     ARGCHK(nrows >= 3 && ncols >= 3, "grid dimensions must be >= 3");
+    // the FP32 row-window sweeps address a source's fields and the FP64 velocities with
+    // 32-bit byte offsets
+    ARGCHK((long)nrows * ncols <= (1L << 28), "2D grid too large (at most 2^28 nodes)");
     ARGCHK(dx > 0 && dz > 0, "spacings must be positive");
     ARGCHK(NumofR >= 0 && (NumofR == 0 || Z), "reflectors need Z");
     HIPCHK(hipSetDevice(ctx->device));

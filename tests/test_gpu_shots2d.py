@@ -181,3 +181,16 @@ def test_row_window_matches_plane_loop_bitwise(ctx, seed, NumofR, shape, nsrc):
     assert a.keys() == b.keys()
     for k in a:
         assert np.array_equal(np.asarray(a[k]).view(np.uint8), np.asarray(b[k]).view(np.uint8)), k
+
+
+def test_model2d_size_bound(ctx):
+    """A 2D grid above 2^28 nodes is refused when the model is created, before the
+    velocities are read: the row-window sweeps address a source's fields and the FP64
+    velocities with 32-bit byte offsets."""
+    import ctypes as C
+    from pfast_amd import _lib
+    L = _lib.lib()
+    vp = np.full((4, 4), 2000.0)
+    out = C.c_void_p()
+    rc = L.pfast_model2d_create(ctx.h, vp, 16385, 16384, 10.0, 10.0, None, 0, C.byref(out))
+    assert rc != 0 and b"too large" in L.pfast_last_error()
