@@ -10,7 +10,8 @@ own FP32 picks; the shot sets run on the starting (1D gradient) model.
 - zero residuals (picks equal to the model's own) give exactly zero gradients and misfits
   (src/RT_padjoint2d.h: the boundary values are the residuals);
 - the adjoints are linear in the residuals: doubled residuals double gT and gR;
-- a source's results do not depend on the other sources of the shot set.
+- a source's results do not depend on the other sources of the shot set;
+- the row-window form and the plane loop give the same bits at this size.
 """
 import numpy as np
 import pytest
@@ -140,3 +141,32 @@ def test_fullsize2d_result_independent_of_other_sources(full2d):
         for which in ("T", "TTu", "lambdaT", "lambdaR"):
             assert np.array_equal(part.field(k, which), sh.field(s, which)), (s, which)
     part.close()
+
+
+def test_fullsize2d_row_window_matches_plane_loop_bitwise(full2d):
+    """At the full size (151 rows: three waves of row threads exchanging through LDS, the
+    reflection and transmission solves in blocks of their own) the row-window form and
+    the plane loop agree bit for bit on every field, gradient and misfit."""
+    pf, cfg, m0, sh, (tt, tr) = full2d
+    sel = np.arange(3, 80, 10)
+    nrec = int(cfg["srnum"][0])
+    rows = np.concatenate([np.arange(s * nrec, (s + 1) * nrec) for s in sel])
+    res = []
+    for form in (0, 1):
+        m0.ctx.set_sweep2d(form)
+        part = pf.Shots2D(m0, cfg["src"][sel], cfg["srnum"][sel], cfg["rec"][rows],
+                          cfg["nxy"][rows], tt[rows], tr[rows])
+        part.run(adjoint=True, MAXSwp=500)
+        out = dict(zip(("gT", "gR", "gRT"), part.gradient(0.5)[:3]))
+        out["E"] = np.array(part.gradient(0.5)[3])
+        out["info"] = np.asarray(part.info()[0])
+        for k in range(len(sel)):
+            for which in ("T", "lambdaT", "lambdaR"):
+                out[f"{which}{k}"] = part.field(k, which)
+            out[f"TTd{k}"] = part.field(k, "TTd", 0)
+            out[f"TTu{k}"] = part.field(k, "TTu", 0)
+        part.close()
+        res.append(out)
+    m0.ctx.set_sweep2d(0)
+    for k in res[0]:
+        assert np.array_equal(res[0][k].view(np.uint8), res[1][k].view(np.uint8)), k
