@@ -48,7 +48,10 @@ def _run(exe, tmp, mode, shape, tile, T, U0, src):
 CASES = [((11, 19, 13), (8, 8, 4)), ((9, 8, 8), (8, 8, 2)), ((13, 21, 17), (4, 4, 4)),
               ((7, 5, 23), (8, 4, 6)), ((12, 16, 9), (8, 8, 4)), ((3, 17, 9), (8, 8, 4)),
               ((10, 14, 11), (4, 4, 6)), ((8, 13, 19), (4, 8, 6)), ((9, 21, 35), (8, 16, 6)),
-              ((7, 37, 18), (16, 8, 4))]
+              ((7, 37, 18), (16, 8, 4)),
+              # the default adjoint batch depth with partial edge tiles and depths that are
+              # not a multiple of two batches
+              ((14, 23, 19), (8, 8, 2)), ((5, 17, 26), (8, 8, 2)), ((21, 9, 30), (8, 8, 2))]
 
 
 @pytest.mark.parametrize("shape,tile", CASES)
