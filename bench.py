@@ -193,6 +193,48 @@ def tomo2d(args):
     return res
 
 
+def driver2d(args):
+    """Wall time of the drop-in 2D driver (pfast_amd/bin/pfast_rt2d, built by build()) on
+    the reference's shipped joint inversion example (tests/golden/example2d_inv.npz:
+    160x600, 56 sources x 600 receivers, two reflectors), RTtag 3, L-BFGS memory 50,
+    10 iterations, from process start to exit (file writes included)."""
+    import re
+    import subprocess
+    import tempfile
+    import numpy as np
+    from pfast_amd import formats as F
+    e = np.load(os.path.join(ROOT, "tests", "golden", "example2d_inv.npz"))
+    nr, nc = (int(v) for v in e["grid"])
+    dx, dz = (float(v) for v in e["spacing"])
+    n = len(e["tt"])
+    rec = np.concatenate([np.tile(e["geom"], (n // len(e["geom"]), 1)),
+                          e["tt"].astype(np.float64)[:, None], np.zeros((n, 1)),
+                          e["tr2"].astype(np.float64)[:, None]], 1)
+    with tempfile.TemporaryDirectory() as d:
+        F.write_bin(os.path.join(d, "m.vp"), e["vp"].astype(np.float64))
+        F.write_bin(os.path.join(d, "m.src"), e["src"])
+        F.write_bin(os.path.join(d, "m.rec"), rec)
+        F.write_bin(os.path.join(d, "m.ref"), e["ref"])
+        F.write_bin(os.path.join(d, "m.wr"), np.full((nr, nc), float(e["wr"][0])))
+        F.write_inp2d(os.path.join(d, "m.inp"), "m.vp", "m.src", e["src"].shape, "m.rec",
+                      rec.shape, "m.ref", e["ref"].shape, nr, nc, dx, dz, vpfile="final.vp",
+                      nux=float(e["nu"][0]), nuz=float(e["nu"][1]), twfile="m.wr", schemetag=3,
+                      lsrc=2)
+        t0 = time.perf_counter()
+        r = subprocess.run([os.path.join(ROOT, "pfast_amd", "bin", "pfast_rt2d"), "m.inp", "50",
+                            "3", "--max-iter", "10", "--no-shot-gather"], cwd=d,
+                           capture_output=True, text=True, timeout=300)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-1000:] + r.stderr[-1000:])
+    res = [float(v) for v in re.findall(r"Residual \(TR\): ([-0-9.eE+]+) ms", r.stdout)]
+    log(f"2D driver: 10 iterations in {wall:.2f} s, residual {res[0]:.4f} -> {res[-1]:.4f} ms")
+    return {"workload": "shipped 2D joint inversion example, 160x600, 56 sources x 600 "
+                        "receivers, 2 reflectors, RTtag 3, L-BFGS m=50",
+            "iterations": 10, "wall_s": round(wall, 3),
+            "residual_TR_ms": [res[0], res[-1]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +269,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-2d", action="store_true",
                     help="skip the 2D tomography and joint T+R gradient timings")
+    ap.add_argument("--driver2d", action="store_true",
+                    help="also time the drop-in 2D driver on the shipped inversion example")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -382,6 +426,12 @@ def main():
         except Exception as exc:  # the headline line must still be printed
             log(f"2D measurement failed: {exc!r}")
             out["tomo2d"] = {"error": repr(exc)}
+    if args.driver2d:
+        try:
+            out["driver2d"] = driver2d(args)
+        except Exception as exc:  # the headline line must still be printed
+            log(f"2D driver measurement failed: {exc!r}")
+            out["driver2d"] = {"error": repr(exc)}
     if not args.no_serve:
         try:
             out["serving"] = serving(args)
