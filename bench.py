@@ -235,6 +235,37 @@ def driver2d(args):
             "residual_TR_ms": [res[0], res[-1]]}
 
 
+def driver3d(cfg, iters=3):
+    """Wall time of the drop-in 3D driver (pfast_amd/bin/pfast_t3d) on this workload:
+    the model above as the starting model, its 400 x 2000 picks, L-BFGS memory 5,
+    `iters` iterations from process start to exit (reading the .inp/.vp/.src/.rec files,
+    every gradient evaluation and line-search trial, the FFT regularisation and the
+    model writes included)."""
+    import re
+    import subprocess
+    import tempfile
+    from pfast_amd import formats as F
+    nz, nr, nc = cfg["vp"].shape
+    h = cfg["h"]
+    with tempfile.TemporaryDirectory() as d:
+        F.write_bin(os.path.join(d, "m.vp"), cfg["vp"])
+        F.write_bin(os.path.join(d, "m.src"), cfg["src_tab"])
+        F.write_bin(os.path.join(d, "m.rec"), cfg["rec_tab"])
+        F.write_inp3d(os.path.join(d, "m.inp"), "m.vp", "m.src", cfg["src_tab"].shape, "m.rec",
+                      cfg["rec_tab"].shape, nr, nc, nz, h, h, h, vpfile="final.vp")
+        t0 = time.perf_counter()
+        r = subprocess.run([os.path.join(ROOT, "pfast_amd", "bin", "pfast_t3d"), "m.inp", "5",
+                            "--max-iter", str(iters)], cwd=d, capture_output=True, text=True,
+                           timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-1000:] + r.stderr[-1000:])
+    trials = len(re.findall(r"\(alpha0,alpha1,phi0,phi1,dphi0,dphi1\)", r.stdout))
+    log(f"3D driver: {iters} iterations in {wall:.2f} s ({trials} line-search trial lines)")
+    return {"workload": "this 3D workload as the starting model, L-BFGS m=5",
+            "iterations": iters, "wall_s": round(wall, 3), "trial_lines": trials}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +302,8 @@ def main():
                     help="skip the 2D tomography and joint T+R gradient timings")
     ap.add_argument("--driver2d", action="store_true",
                     help="also time the drop-in 2D driver on the shipped inversion example")
+    ap.add_argument("--driver3d", action="store_true",
+                    help="also time 3 iterations of the drop-in 3D driver on this workload")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -426,6 +459,12 @@ def main():
         except Exception as exc:  # the headline line must still be printed
             log(f"2D measurement failed: {exc!r}")
             out["tomo2d"] = {"error": repr(exc)}
+    if args.driver3d:
+        try:
+            out["driver3d"] = driver3d(cfg)
+        except Exception as exc:  # the headline line must still be printed
+            log(f"3D driver measurement failed: {exc!r}")
+            out["driver3d"] = {"error": repr(exc)}
     if args.driver2d:
         try:
             out["driver2d"] = driver2d(args)

@@ -23,5 +23,5 @@ def test_options():
     r = _bench("--help")
     assert r.returncode == 0
     for opt in ("--gpus", "--steps", "--warmup", "--sweep", "--autotune", "--streams",
-                "--no-2d", "--no-serve", "--no-cpu", "--copy-calibration", "--driver2d"):
+                "--no-2d", "--no-serve", "--no-cpu", "--copy-calibration", "--driver2d", "--driver3d"):
         assert opt in r.stdout, opt
