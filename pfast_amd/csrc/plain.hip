@@ -656,7 +656,9 @@ __device__ int wg_adj2d(typename P::real *__restrict__ U, typename P::real *__re
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
             if constexpr (!P::exact) {
-                if (g.window && blockDim.x == W2NT && nri >= 1 && nri <= W2NT && nci >= 1) {
+                // the window keeps one halo-row queue per thread, so it needs two
+                // interior rows at least (one row would be both first and last)
+                if (g.window && blockDim.x == W2NT && nri >= 2 && nri <= W2NT && nci >= 1) {
                     if (adj2d_window(U, T, tag, g, s1, s2)) *s_flag = 1;
                     __syncthreads();
                     continue;

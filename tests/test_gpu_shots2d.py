@@ -44,6 +44,7 @@ def rel_l2(a, b):
     (61, 0, (22, 31), 4),
     (62, 1, (26, 37), 5),
     (63, 2, (30, 41), 6),
+    (64, 0, (3, 40), 4),  # one interior row: the adjoint takes the plane loop
 ])
 def test_shots2d_vs_oracle(ctx, seed, NumofR, shape, nsrc):
     import pfast_amd
@@ -157,6 +158,8 @@ def _all_results(sh, nsrc, R):
     (71, 0, (23, 35), 5),
     (72, 1, (29, 44), 6),
     (73, 2, (31, 47), 7),
+    (74, 0, (3, 36), 4),
+    (75, 0, (4, 30), 3),
 ])
 def test_row_window_matches_plane_loop_bitwise(ctx, seed, NumofR, shape, nsrc):
     """The row-window sweeps (a work-group walking the rows, independent solves of a
