@@ -287,6 +287,10 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="choose the sweep variant by timing every variant on this shot set "
                          "first (results are the same under each)")
+    ap.add_argument("--schedule", choices=["auto", "dataflow", "diagonal"], default="auto",
+                    help="batched 3D sweep schedule: dataflow launches (two per cycle), one "
+                         "launch per tile anti-diagonal, or per cycle the one that suits its "
+                         "active chunks (same results)")
     ap.add_argument("--no-tile-skip", action="store_true",
                     help="sweep every tile (exact tile skipping off; same results)")
     ap.add_argument("--streams", type=int, default=3,
@@ -338,6 +342,7 @@ def main():
     ctx = pfast_amd.Context(0)
     ctx.set_streams(args.streams)
     ctx.set_tile_skip(not args.no_tile_skip)
+    ctx.set_dataflow({"auto": 2, "dataflow": 1, "diagonal": 0}[args.schedule])
     ctx.set_sweep(args.sweep)
     sweep_name = pfast_amd.sweep_variants()[args.sweep]
     log(f"sweep kernels: {sweep_name}")
@@ -385,7 +390,9 @@ def main():
     adj_bytes = adj_units * 12.0             # lambda read+write, T read
     fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
     adj_gbs = adj_bytes / (adj_ms * 1e-3) / 1e9
-    kfwd, kadj = "fwd3d_col", "adj3d_col"
+    kfwd, kadj = {"dataflow": ("fwd3d_flow", "adj3d_flow"),
+                  "diagonal": ("fwd3d_col", "adj3d_col"),
+                  "auto": ("fwd3d_col+fwd3d_flow", "adj3d_col+adj3d_flow")}[args.schedule]
     dom = kadj if adj_ms >= fwd_ms else kfwd
     log(f"kernels: fwd {fwd_ms:.1f} ms / {fwd_n} launches -> {fwd_gbs:.0f} GB/s; "
         f"adj {adj_ms:.1f} ms / {adj_n} launches -> {adj_gbs:.0f} GB/s")
@@ -429,7 +436,9 @@ def main():
                                f"{args.nsrc} sources x {args.nrec} receivers, forward+adjoint+gradient",
                    "nsrc": args.nsrc, "nrec": args.nrec, "grid": [101, 201, 201], "h": h,
                    "parallelism": "all sources batched on 1 GPU", "sweep": sweep_name,
-                   "streams": args.streams, "tile_skip": not args.no_tile_skip},
+                   "schedule": args.schedule,
+                   "streams": args.streams,
+                   "tile_skip": not args.no_tile_skip},
         "roofline": roofline,
         "kernels": {kfwd: {"ms": round(fwd_ms, 2), "launches": fwd_n, "GBps": round(fwd_gbs, 1)},
                     kadj: {"ms": round(adj_ms, 2), "launches": adj_n, "GBps": round(adj_gbs, 1)}},

@@ -182,6 +182,14 @@ int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams);
  * sweep in an ordering changed nothing, and whose nodes and neighbour tiles have
  * not changed since, is not swept again in that ordering; results are the same. */
 int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable);
+/* Schedule of the batched 3D sweep cycles (the loops of src/T_pfsm3d.h:152-179 and
+ * src/T_padjoint3d.h:135-163, same node updates in the same order under every mode):
+ * 0 = one launch per tile anti-diagonal and ordering; 1 = dataflow launches, two per
+ * cycle, in which a tile starts as soon as the values it reads are final (tiles of
+ * later diagonals and orderings overlap, level by level); 2 (default) = per cycle,
+ * dataflow when the active chunks x tiles of an ordering are few enough that the
+ * per-diagonal launches would leave the GPU idle.  Results are the same. */
+int pfast_ctx_set_dataflow(pfast_ctx *ctx, int mode);
 
 /* ---------------------------------------------------------------------------------
  * Batched 2D shot sets: every source of a 2D survey resident on one GPU (FP32

@@ -175,6 +175,14 @@ class Context:
         """Exact tile skipping of the batched 3D sweeps (default on; same results)."""
         check(lib().pfast_ctx_set_tile_skip(self.h, 1 if enable else 0))
 
+    def set_dataflow(self, mode=True):
+        """Schedule of the batched 3D sweep cycles: False/0 one launch per tile
+        anti-diagonal, True/1 dataflow launches (a tile starts as soon as the values it
+        reads are final), "auto"/2 (default) dataflow for cycles with few chunks x
+        tiles.  Results are the same under every mode."""
+        m = 2 if mode == "auto" else int(mode)
+        check(lib().pfast_ctx_set_dataflow(self.h, m))
+
     def set_streams(self, n):
         """Split batched 3D cycles over n streams (1..4); results do not change."""
         check(lib().pfast_ctx_set_streams(self.h, int(n)))

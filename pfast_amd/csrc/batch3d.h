@@ -37,6 +37,20 @@ struct SweepArgs {
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
 };
 
+// State of a dataflow sweep launch (the four orderings of one depth direction of a
+// cycle over every active chunk; flow.h has the schedule).
+struct FlowArgs {
+    unsigned *prog;      // [chunk id][8][tile] progress words: (epoch << 16) | steps
+    unsigned nprog;      // progress words; the counter, abort and all-ones words follow
+    unsigned *ticket;    // task claim counter (back to 0 when the launch's last block claims)
+    int *abort;          // set when a dependency wait times out (results void)
+    unsigned base;       // this launch's epoch << 16 (epochs increase within a phase)
+    unsigned ntasks;     // 4 * tiles * active chunks
+    int o0;              // first ordering: 0 (depth decreasing) or 4 (increasing)
+    int cycle, nd;       // cycle number and tile diagonals, for the tile-skip epochs
+    unsigned long long timeout;  // wall-clock ticks a dependency wait may take
+};
+
 // A sweep kernel variant: schedule (0: column window, the only one), tile rows x
 // columns, batch depth of the forward and of the adjoint kernel.
 struct SweepVariant {
@@ -49,6 +63,18 @@ int sweep_variant_count();
 const SweepVariant &sweep_variant(int v);
 hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int adjoint,
                           int uniform, int D, int s1, int s2, int s3, int ntiles_diag);
+// One dataflow launch (a.active / a.nact: the chunks; f.o0: which half of the cycle;
+// variant as launch_sweep3d).
+hipError_t launch_sweep3d_flow(hipStream_t st, const SweepArgs &a, const FlowArgs &f,
+                               int variant, int adjoint, int uniform);
+// Resident work-groups of a dataflow kernel on the current device (the grid size).
+int flow_grid(int variant, int adjoint, int uniform);
+// Words of the dataflow state of `nchunk` chunks of a grid of `ntile` tiles: progress
+// words, then the claim counter, a spare word, the abort word and an all-ones word.
+size_t flow_words(int nchunk, int ntile);
+// Reset the dataflow state (the start of a phase); a kernel, so that it may sit in a
+// captured graph.
+hipError_t launch_flow_reset(hipStream_t st, unsigned *state, size_t words);
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
 hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,
                            int npick, long N, int unmark);

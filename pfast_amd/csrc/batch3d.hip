@@ -17,8 +17,13 @@
 // sweep-ordered position), so every node sees its behind neighbours updated and
 // its ahead neighbours not yet, as in the reference's lexicographic sweep.
 #include "batch3d.h"
+#include "flow.h"
 #include "pfast_internal.h"
 #include "solvers.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace {
 
@@ -163,20 +168,22 @@ __device__ __forceinline__ int skip_index(const SweepArgs &a, const TileCtx &t)
 
 // The sweep of tile ix in this ordering would change nothing (see SweepArgs): its
 // clean sweep is newer than every change of the tile and of the neighbour faces
-// it reads (left tile's x+ face, right's x-, upper's y+, lower's y-).
-template <int TI, int TJ>
+// it reads (left tile's x+ face, right's x-, upper's y+, lower's y-).  DEV:
+// device-scope reads, for records written earlier in the same launch.
+template <int TI, int TJ, bool DEV = false>
 __device__ __forceinline__ bool skip_tile(const SweepArgs &a, const TileCtx &t, int ix)
 {
     if (ix < 0) return false;
-    const int cl = a.tclean[ix * 8 + a.o];
+    auto ld = [](const int *p) { return DEV ? ld_agent(p) : *p; };
+    const int cl = ld(&a.tclean[ix * 8 + a.o]);
     if (cl <= 0) return false;
     const int ti = t.i0 / TI, tj = t.j0 / TJ;
     const int *M = a.tmod;
-    int m = M[ix * 5];
-    if (tj > 0) m = max(m, M[(ix - 1) * 5 + 2]);
-    if (tj < a.ntj - 1) m = max(m, M[(ix + 1) * 5 + 1]);
-    if (ti > 0) m = max(m, M[(ix - a.ntj) * 5 + 4]);
-    if (ti < a.nti - 1) m = max(m, M[(ix + a.ntj) * 5 + 3]);
+    int m = ld(&M[ix * 5]);
+    if (tj > 0) m = max(m, ld(&M[(ix - 1) * 5 + 2]));
+    if (tj < a.ntj - 1) m = max(m, ld(&M[(ix + 1) * 5 + 1]));
+    if (ti > 0) m = max(m, ld(&M[(ix - a.ntj) * 5 + 4]));
+    if (ti < a.nti - 1) m = max(m, ld(&M[(ix + a.ntj) * 5 + 3]));
     return m < cl;
 }
 
@@ -202,49 +209,146 @@ __device__ __forceinline__ void record_tile(const SweepArgs &a, int ix, unsigned
     if (threadIdx.x == 0) {
         const int v = *any;
         if (v) {
-            a.tmod[ix * 5] = a.epoch;
+            st_agent(&a.tmod[ix * 5], a.epoch);
 #pragma unroll
             for (int f = 0; f < 4; f++)
-                if ((v >> f) & 1) a.tmod[ix * 5 + 1 + f] = a.epoch;
-            a.tclean[ix * 8 + a.o] = 0;
+                if ((v >> f) & 1) st_agent(&a.tmod[ix * 5 + 1 + f], a.epoch);
+            st_agent(&a.tclean[ix * 8 + a.o], 0);
         } else {
-            a.tclean[ix * 8 + a.o] = a.epoch;
+            st_agent(&a.tclean[ix * 8 + a.o], a.epoch);
         }
     }
 }
 
-template <int TI, int TJ, int C, int L, bool UNI>
-__global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s1, int s2, int s3)
+// ------------------------------------------------------------ dataflow waits
+// Dependency wait of a dataflow task (flow.h).  Wave 0 of the block watches the
+// task's dependencies -- lane d < ndep dependency d -- and publishes in LDS the last
+// step whose memory operations they allow; the other waves read that word and never
+// poll memory.  `dv` holds each word as read one batch earlier, so the check costs no
+// memory latency once the dependencies are ahead.
+struct FlowWait {
+    rsrc_t R;            // the launch's dataflow state words
+    unsigned w;          // this lane's dependency: word index << 6 | requirement code
+    unsigned dv;         // the word as last read (wave 0)
+    unsigned base;       // launch epoch << 16
+    int smax_cap;        // pend - 1: later steps touch no memory
+    unsigned abort_off;  // byte offset of the abort word
+    volatile int *ok;    // LDS: steps up to *ok are allowed
+    bool watcher;        // this wave is wave 0
+    unsigned long long timeout;  // wall-clock ticks a wait may take
+};
+
+// Requirement codes: pflow::need_add values below kCodeDone, kCodeDone for kDone.
+constexpr unsigned kCodeDone = 63;
+constexpr int kOkAll = 0x7fffffff;
+
+// The last step up to which every dependency (as last read) allows memory operations:
+// a word of an earlier launch allows nothing, a finished task everything, else steps
+// completed minus the requirement's offset (flow.h).  Wave 0, wave-uniform result.
+__device__ __forceinline__ int flow_ok_until(const FlowWait &f)
+{
+    const unsigned code = f.w & 63u;
+    int v;
+    if (f.dv < f.base) v = -1;
+    else if (f.dv - f.base >= pflow::kDone) v = kOkAll;
+    else if (code == kCodeDone) v = -1;
+    else v = (int)(f.dv - f.base) - (int)code;
+    int m = __builtin_amdgcn_readlane(v, 0);
+#pragma unroll
+    for (int l = 1; l < pflow::kMaxDeps; l++) m = min(m, __builtin_amdgcn_readlane(v, l));
+    return m;
+}
+
+__device__ __forceinline__ void flow_prefetch(FlowWait &f)
+{
+    if (f.watcher)
+        f.dv = __builtin_amdgcn_raw_buffer_load_b32(f.R, (f.w >> 6) * 4u, 0, kCpolDevice);
+}
+
+// Until the task's dependencies allow the memory operations of steps up to smax
+// (wave-uniform).  A wait longer than the timeout sets the abort word, and a wait that
+// sees it set gives up: the task then runs to its end without waiting (the launch's
+// results are void and the host reports it), so every block leaves.
+__device__ __forceinline__ void flow_wait(FlowWait &f, int smax)
+{
+    smax = min(smax, f.smax_cap);
+    if (f.watcher) {
+        int ok = flow_ok_until(f);
+        if (ok < smax) {
+            const unsigned long long t0 = wall_clock64();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(4);
+                flow_prefetch(f);
+                ok = flow_ok_until(f);
+                if (ok >= smax) break;
+                if (__builtin_amdgcn_raw_buffer_load_b32(f.R, f.abort_off, 0, kCpolDevice)) {
+                    ok = kOkAll;
+                    break;
+                }
+                if (wall_clock64() - t0 > f.timeout) {
+                    __builtin_amdgcn_raw_buffer_store_b32(1u, f.R, f.abort_off, 0, kCpolDevice);
+                    ok = kOkAll;
+                    break;
+                }
+            }
+        }
+        if ((threadIdx.x & 63) == 0 && ok > *f.ok) *f.ok = ok;
+    } else {
+        while (*f.ok < smax) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Field access of the tile sweeps.  The per-diagonal kernels read only what earlier
+// launches wrote and use the default cache policy; the dataflow kernels read what
+// other work-groups of the same launch wrote, possibly on another XCD, and access the
+// fields at device scope.
+template <bool DEV>
+__device__ __forceinline__ float fload(rsrc_t r, unsigned off)
+{
+    return DEV ? bload_dev(r, off) : bload(r, off);
+}
+template <bool DEV>
+__device__ __forceinline__ float2 fload2(rsrc_t r, unsigned off)
+{
+    return DEV ? bload2_dev(r, off) : bload2(r, off);
+}
+template <bool DEV>
+__device__ __forceinline__ void fstore(rsrc_t r, unsigned off, float v)
+{
+    if (DEV) bstore_dev(r, off, v);
+    else bstore(r, off, v);
+}
+
+// ======================================================= column-window tiles
+// One tile sweep of the forward column window (the body of fwd3d_col and of each
+// task of fwd3d_flow).  FLOW: wait on the task's dependencies before each batch's
+// memory operations and publish the task's progress every batch (flow.h).
+template <int TI, int TJ, int C, int L, bool UNI, bool FLOW>
+__device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
+                                         int s3, int six, float *pub, int *any_change,
+                                         FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
     static_assert(C % 4 == 0, "a node's sources fill whole DPP quads");
     // per step parity: [m-1 | m+1][thread], then an INF slot (domain-edge neighbour)
     constexpr int PB = 2 * NT + 1;
-    __shared__ float pub[2 * PB];
-
-    __shared__ int any_change;
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const int six = skip_index<TI, TJ>(a, t);
-    if (skip_tile<TI, TJ>(a, t, six)) return;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
     const unsigned fbytes = 4u * plane * (unsigned)nz * C;
-    const rsrc_t RF = make_rsrc(a.chunks[t.chunk].field[0], fbytes);
+    const rsrc_t RF = make_rsrc(uniform_ptr(a.chunks[t.chunk].field[0]), fbytes);
     // 8x8 tiles read the tile-major cell times: a tile's level is 256 contiguous bytes
     // (two cache lines) instead of eight 32-byte pieces of grid rows
     constexpr bool F8 = TI == 8 && TJ == 8;
     const unsigned tplane = (unsigned)a.nti * a.ntj * 64u;
-    const rsrc_t RS = F8 ? make_rsrc(a.chunks[t.chunk].slow8, 4u * tplane * (unsigned)nz)
-                         : make_rsrc(a.chunks[t.chunk].slow, 4u * plane * (unsigned)nz);
+    const rsrc_t RS = F8 ? make_rsrc(uniform_ptr(a.chunks[t.chunk].slow8), 4u * tplane * (unsigned)nz)
+                         : make_rsrc(uniform_ptr(a.chunks[t.chunk].slow), 4u * plane * (unsigned)nz);
     const float INF = PFAST_INF_F;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = INF;
-        any_change = 0;
+        *any_change = 0;
     }
     // steps p at which this thread updates its node: pipj <= p < pipj + nz, except
     // the level of its own source
@@ -280,19 +384,19 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
         const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
 #pragma unroll
         for (int u = 0; u < L; u++) {
-            const float o = bload(RF, l.oI + dm + (unsigned)(u + 2) * st);
+            const float o = fload<FLOW>(RF, l.oI + dm + (unsigned)(u + 2) * st);
             ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF;
         }
         if (hxuse) {
 #pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = bload(RF, l.oHX + dm + (unsigned)u * st);
+            for (int u = 0; u < L; u++) hx[u] = fload<FLOW>(RF, l.oHX + dm + (unsigned)u * st);
         } else {
 #pragma unroll
             for (int u = 0; u < L; u++) hx[u] = INF;
         }
         if (hyuse) {
 #pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = bload(RF, l.oHY + dm + (unsigned)u * st);
+            for (int u = 0; u < L; u++) hy[u] = fload<FLOW>(RF, l.oHY + dm + (unsigned)u * st);
         } else {
 #pragma unroll
             for (int u = 0; u < L; u++) hy[u] = INF;
@@ -300,13 +404,18 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 #pragma unroll
         for (int g = 0; g < FQ; g++) fv[g] = bload(RS, oFc + dmf + (unsigned)(4 * g) * stF);
     };
+    if constexpr (FLOW) {
+        flow_wait(*fw, L - 1);
+        asm volatile("" ::: "memory");
+    }
     // window at step 0 (level m0 = -pipj); level m0-1 is outside the column
     const int m0 = -l.pipj;
     float vprev = INF;
-    float vcur = l.valid && m0 == 0 ? bload(RF, l.oI) : INF;
-    float vnext = l.valid && m0 >= -1 ? bload(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
+    float vcur = l.valid && m0 == 0 ? fload<FLOW>(RF, l.oI) : INF;
+    float vnext = l.valid && m0 >= -1 ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
     float qa[4][L], qb[4][L];
     load_batch(0, qa[0], qa[1], qa[2], qa[3]);
+    if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();  // INF slot visible
 
     bool changed = false;
@@ -321,15 +430,20 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
     // of the following batch into nq first
     auto batch = [&](int pb, float (*q)[L], float (*nq)[L]) {
         wait_vm_all();
+        if constexpr (FLOW) {
+            flow_wait(*fw, pb + 2 * L - 1);
+            flow_prefetch(*fw);
+        }
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
-        asm volatile("" : "+s"(pbo));
+        if constexpr (FLOW) asm volatile("" : "+s"(pbo) : : "memory");
+        else asm volatile("" : "+s"(pbo));
         {  // changed nodes of the previous batch (steps pb-L .. pb-1)
             const unsigned d = (unsigned)(pbo - L - l.pipj) * st;
 #pragma unroll
             for (int u = 0; u < L; u++)
-                if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
+                if (chg[u]) fstore<FLOW>(RF, l.oI + d + (unsigned)u * st, sval[u]);
         }
         load_batch(pbo + L, nq[0], nq[1], nq[2], nq[3]);
 #pragma unroll
@@ -339,6 +453,10 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
             P[tid] = vprev;
             P[NT + tid] = vnext;
             lds_barrier();
+            if constexpr (FLOW) {
+                // every wave has waited for its stores of steps < pb - L
+                if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
+            }
             const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
             const float xa = fminf(l.uxm ? q[1][u] : pxm, l.uxp ? q[1][u] : pxp);
             const float ya = fminf(l.uym ? q[2][u] : pym, l.uyp ? q[2][u] : pyp);
@@ -375,12 +493,26 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
         const unsigned d = (unsigned)(pl - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++)
-            if (chg[u]) bstore(RF, l.oI + d + (unsigned)u * st, sval[u]);
+            if (chg[u]) fstore<FLOW>(RF, l.oI + d + (unsigned)u * st, sval[u]);
     }
+    if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
     record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
-                &any_change);
+                any_change);
+}
+
+template <int TI, int TJ, int C, int L, bool UNI>
+__global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s1, int s2, int s3)
+{
+    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
+    __shared__ float pub[2 * PB];
+    __shared__ int any_change;
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const int six = skip_index<TI, TJ>(a, t);
+    if (skip_tile<TI, TJ>(a, t, six)) return;
+    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr, nullptr);
 }
 
 // Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
@@ -388,27 +520,21 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // marks a fixed receiver node), and a changed node stores its lambda alone.  Record
 // offsets are twice the traveltime field's.  Only interior nodes are updated, and
 // every neighbour they read is inside the domain.
-template <int TI, int TJ, int C, int L, bool S3POS>
-__global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
+template <int TI, int TJ, int C, int L, bool S3POS, bool FLOW>
+__device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
+                                         int six, float2 *pub, int *any_change, FlowWait *fw,
+                                         unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     constexpr int s3 = S3POS ? 1 : -1;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
     constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][thread], INF slot
-    __shared__ float2 pub[2 * PB];
-
-    __shared__ int any_change;
-
-    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
-    const int six = skip_index<TI, TJ>(a, t);
-    if (skip_tile<TI, TJ>(a, t, six)) return;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
     const unsigned plane = (unsigned)a.nrows * a.ncols;
     const unsigned fbytes = 8u * plane * (unsigned)nz * C;
-    const rsrc_t RLT = make_rsrc(a.chunks[t.chunk].field[1], fbytes);
+    const rsrc_t RLT = make_rsrc(uniform_ptr(a.chunks[t.chunk].field[1]), fbytes);
     // 8-byte records: the traveltime field's offsets doubled, mod 2^32 (the shot set's
     // size bound, 8 (N + 64 nrows ncols) C <= 2^32, keeps a level outside the column
     // outside the field)
@@ -422,13 +548,13 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     const float wx = a.wx, wy = a.wy, wz = a.wz;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
-        any_change = 0;
+        *any_change = 0;
     }
 
     // halo loads only in the lanes that use them (vector-memory instructions bound
     // the sweep; a wave without a halo lane skips them)
     const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
-    auto ld2 = [&](unsigned o) { return bload2(RLT, o); };
+    auto ld2 = [&](unsigned o) { return fload2<FLOW>(RLT, o); };
     auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
         const unsigned dm = (unsigned)(pb - l.pipj) * st;
 #pragma unroll
@@ -448,12 +574,17 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
             for (int u = 0; u < L; u++) hy[u] = make_float2(0.0f, 0.0f);
         }
     };
+    if constexpr (FLOW) {
+        flow_wait(*fw, L - 1);
+        asm volatile("" ::: "memory");
+    }
     const int m0 = -l.pipj;
     float2 vprev = make_float2(0.0f, 0.0f);
     float2 vcur = l.valid && m0 == 0 ? ld2(oI) : make_float2(0.0f, 0.0f);
     float2 vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
     float2 qa[3][L], qb[3][L];
     load_batch(0, qa[0], qa[1], qa[2]);
+    if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();
 
     bool changed = false;
@@ -466,15 +597,20 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     }
     auto batch = [&](int pb, float2 (*q)[L], float2 (*nq)[L]) {
         wait_vm_all();
+        if constexpr (FLOW) {
+            flow_wait(*fw, pb + 2 * L - 1);
+            flow_prefetch(*fw);
+        }
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
-        asm volatile("" : "+s"(pbo));
+        if constexpr (FLOW) asm volatile("" : "+s"(pbo) : : "memory");
+        else asm volatile("" : "+s"(pbo));
         {
             const unsigned d = (unsigned)(pbo - L - l.pipj) * st;
 #pragma unroll
             for (int u = 0; u < L; u++)
-                if (chg[u]) bstore(RLT, oI + d + (unsigned)u * st, sval[u]);
+                if (chg[u]) fstore<FLOW>(RLT, oI + d + (unsigned)u * st, sval[u]);
         }
         load_batch(pbo + L, nq[0], nq[1], nq[2]);
 #pragma unroll
@@ -484,6 +620,9 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
             P[tid] = vprev;
             P[NT + tid] = vnext;
             lds_barrier();
+            if constexpr (FLOW) {
+                if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
+            }
             // branch-free: the (lambda, T) pairs of the neighbours come in one LDS read
             // each, the update is computed in every lane and selected afterwards
             const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
@@ -535,12 +674,212 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
         const unsigned d = (unsigned)(pl - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++)
-            if (chg[u]) bstore(RLT, oI + d + (unsigned)u * st, sval[u]);
+            if (chg[u]) fstore<FLOW>(RLT, oI + d + (unsigned)u * st, sval[u]);
     }
+    if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
     record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
-                &any_change);
+                any_change);
+}
+
+template <int TI, int TJ, int C, int L, bool S3POS>
+__global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
+{
+    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
+    __shared__ float2 pub[2 * PB];
+    __shared__ int any_change;
+    const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
+    if (!t.live) return;
+    const int six = skip_index<TI, TJ>(a, t);
+    if (skip_tile<TI, TJ>(a, t, six)) return;
+    adj_tile<TI, TJ, C, L, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr, nullptr);
+}
+
+// ======================================================= dataflow kernels
+// One launch sweeps every active chunk through the 8 orderings of a cycle, one block
+// per task: each block claims the next task number in the order of flow.h as it
+// starts and waits on its dependencies batch by batch, so tiles of later diagonals
+// and orderings start as soon as the values they read are final instead of at a
+// launch boundary.  (A block that runs several tasks in a loop would hold the loop's
+// invariants in registers across the tile sweep: one task per block keeps the sweep
+// near the register count of the per-diagonal kernels.  The kernels are held to 64
+// VGPRs, 8 waves per SIMD; the few values that then spill are set up once per task.)
+
+struct FlowTaskLds {
+    unsigned t;  // claimed task number (>= ntasks: none left)
+    int o, ti, tj, chunk, D, skip;
+    int ok;      // steps whose memory operations the dependencies allow (FlowWait)
+    int ndep;
+    pflow::Dep d[pflow::kMaxDeps];
+};
+
+// A value every lane read from the same LDS word, as a scalar.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Progress word of tile (ti, tj) of a chunk in ordering o.
+__device__ __forceinline__ unsigned *flow_word(const SweepArgs &a, const FlowArgs &f, int chunk,
+                                               int o, int ti, int tj)
+{
+    return f.prog + ((unsigned)chunk * 8u + (unsigned)o) * ((unsigned)a.nti * a.ntj) +
+           (unsigned)ti * a.ntj + (unsigned)tj;
+}
+
+// Thread 0 waits until dependency word w is finished (the exact tile-skip test reads
+// the neighbours' final records).
+__device__ __forceinline__ void flow_wait_done(const FlowArgs &f, const unsigned *w)
+{
+    const unsigned long long t0 = wall_clock64();
+    while (ld_agent(w) < f.base + pflow::kDone) {
+        if (ld_agent(f.abort)) return;
+        if (wall_clock64() - t0 > f.timeout) {
+            st_agent(f.abort, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Claim this block's task into s (all threads).  Blocks take task numbers from the
+// counter in the order they start, not by block index, so a block only ever waits on
+// tasks that started before it; the last claim returns the counter to zero for the
+// next launch.  A task whose sweep would change nothing (exact tile skipping) is
+// finished here (s.skip), as is every task once the launch has aborted.
+template <int TI, int TJ>
+__device__ __forceinline__ void flow_claim(const SweepArgs &a, const FlowArgs &f, FlowTaskLds &s)
+{
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(f.ticket, 1u);
+        if (t == gridDim.x - 1) st_agent(f.ticket, 0u);
+        if (ld_agent(f.abort)) t = f.ntasks;
+        s.t = t;
+        s.ok = -1;
+        if (t < f.ntasks) {
+            const pflow::Task k = pflow::decode(t, f.o0, a.nact, a.nti, a.ntj);
+            s.o = k.o;
+            s.ti = k.ti;
+            s.tj = k.tj;
+            s.D = k.D;
+            s.chunk = a.active[k.slot];
+            const int n = pflow::deps(k.o, f.o0, k.ti, k.tj, a.nti, a.ntj, TI, TJ, a.nrows, a.ncols,
+                                      s.d);
+            s.ndep = n;
+            s.skip = 0;
+            if (a.tmod) {
+                const int ix = (s.chunk * a.nti + k.ti) * a.ntj + k.tj;
+                const int cl = ld_agent(&a.tclean[ix * 8 + k.o]);
+                if (cl > 0 && ld_agent(&a.tmod[ix * 5]) < cl) {
+                    // a candidate: the exact test needs the neighbours' final records
+                    for (int i = 0; i < n; i++)
+                        flow_wait_done(f, flow_word(a, f, s.chunk, s.d[i].o, s.d[i].ti, s.d[i].tj));
+                    SweepArgs at = a;
+                    at.o = k.o;
+                    TileCtx tc;
+                    tc.chunk = s.chunk;
+                    tc.i0 = k.ti * TI;
+                    tc.j0 = k.tj * TJ;
+                    s.skip = skip_tile<TI, TJ, true>(at, tc, ix) ? 1 : 0;
+                }
+            }
+            if (s.skip) st_agent(flow_word(a, f, s.chunk, k.o, k.ti, k.tj), f.base + pflow::kDone);
+        }
+    }
+}
+
+// Per-task setup shared by both dataflow kernels: the tile, the launch arguments of
+// its ordering, this lane's dependency and the task's own progress word.
+template <int TI, int TJ>
+__device__ __forceinline__ void flow_task(const SweepArgs &a, const FlowArgs &f,
+                                          FlowTaskLds &s, TileCtx &t, SweepArgs &at,
+                                          FlowWait &fw, unsigned *&self)
+{
+    const int o = uni(s.o), ti = uni(s.ti), tj = uni(s.tj), chunk = uni(s.chunk);
+    t.live = true;
+    t.chunk = chunk;
+    t.i0 = ti * TI;
+    t.j0 = tj * TJ;
+    t.TIe = min(TI, a.nrows - t.i0);
+    t.TJe = min(TJ, a.ncols - t.j0);
+    t.span = t.TIe + t.TJe - 2;
+    at = a;
+    at.o = o;
+    at.epoch = 1 + ((f.cycle * 8 + o) * f.nd + uni(s.D));
+    self = flow_word(a, f, chunk, o, ti, tj);
+    const int lane = threadIdx.x & 63;
+    const bool use = lane < uni(s.ndep);
+    const pflow::Dep d = s.d[use ? lane : 0];
+    // a lane without a dependency watches the state's all-ones word
+    const unsigned ntile = (unsigned)a.nti * a.ntj;
+    const unsigned words = f.nprog + 4u;
+    const unsigned idx = use ? ((unsigned)chunk * 8u + (unsigned)d.o) * ntile +
+                                   (unsigned)d.ti * a.ntj + (unsigned)d.tj
+                             : words - 1u;
+    const unsigned add = use ? pflow::need_add(d.kind, d.width, TI, TJ) : 0u;
+    fw.R = make_rsrc(f.prog, 4u * words);
+    fw.w = idx << 6 | (add >= pflow::kDone ? kCodeDone : add);
+    fw.base = f.base;
+    fw.dv = 0;
+    fw.abort_off = 4u * (f.nprog + 2u);
+    fw.ok = &s.ok;
+    fw.watcher = uni(threadIdx.x >> 6) == 0;
+    fw.timeout = f.timeout;
+    fw.smax_cap = a.nz + t.span - 1;
+    flow_prefetch(fw);
+}
+
+// Finish a task: every wave's stores are complete (each waited before the block's
+// last barrier); thread 0's record stores complete before the progress word says so.
+__device__ __forceinline__ void flow_finish(const FlowArgs &f, unsigned *self)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        wait_vm_all();
+        st_agent(self, f.base + pflow::kDone);
+    }
+}
+
+template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
+__global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void fwd3d_flow(SweepArgs a, FlowArgs f)
+{
+    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
+    __shared__ float pub[2 * PB];
+    __shared__ int any_change;
+    __shared__ FlowTaskLds s;
+    flow_claim<TI, TJ>(a, f, s);
+    __syncthreads();
+    if ((unsigned)uni((int)s.t) >= f.ntasks || uni(s.skip)) return;
+    TileCtx t;
+    SweepArgs at;
+    FlowWait fw;
+    unsigned *self;
+    flow_task<TI, TJ>(a, f, s, t, at, fw, self);
+    const int o = at.o;
+    const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
+    fwd_tile<TI, TJ, C, L, UNI, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), S3POS ? 1 : -1,
+                                      six, pub, &any_change, &fw, self);
+    flow_finish(f, self);
+}
+
+template <int TI, int TJ, int C, int L, bool S3POS>
+__global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void adj3d_flow(SweepArgs a, FlowArgs f)
+{
+    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
+    __shared__ float2 pub[2 * PB];
+    __shared__ int any_change;
+    __shared__ FlowTaskLds s;
+    flow_claim<TI, TJ>(a, f, s);
+    __syncthreads();
+    if ((unsigned)uni((int)s.t) >= f.ntasks || uni(s.skip)) return;
+    TileCtx t;
+    SweepArgs at;
+    FlowWait fw;
+    unsigned *self;
+    flow_task<TI, TJ>(a, f, s, t, at, fw, self);
+    const int o = at.o;
+    const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
+    adj_tile<TI, TJ, C, L, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
+                                        &any_change, &fw, self);
+    flow_finish(f, self);
 }
 
 // ------------------------------------------------------------------- helpers
@@ -787,6 +1126,87 @@ hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int a
     PFAST_VARIANTS(PFAST_VARIANT_CASE)
 #undef PFAST_VARIANT_CASE
     return hipErrorInvalidValue;
+}
+
+namespace {
+template <int TI, int TJ, int L, int LA, bool S3POS>
+const void *flow_kernel(int adjoint, int uniform)
+{
+    constexpr int C = PFAST_CHUNK;
+    if (adjoint) return reinterpret_cast<const void *>(&adj3d_flow<TI, TJ, C, LA, S3POS>);
+    if (uniform) return reinterpret_cast<const void *>(&fwd3d_flow<TI, TJ, C, L, true, S3POS>);
+    return reinterpret_cast<const void *>(&fwd3d_flow<TI, TJ, C, L, false, S3POS>);
+}
+
+// The dataflow kernel of a variant for the orderings starting at o0 (0 or 4).
+const void *flow_kernel_of(int variant, int adjoint, int uniform, int o0)
+{
+    int v = 0;
+#define PFAST_VARIANT_CASE(TI, TJ, L, LA)                                               \
+    if (variant == v++)                                                                  \
+        return o0 < 4 ? flow_kernel<TI, TJ, L, LA, false>(adjoint, uniform)             \
+                      : flow_kernel<TI, TJ, L, LA, true>(adjoint, uniform);
+    PFAST_VARIANTS(PFAST_VARIANT_CASE)
+#undef PFAST_VARIANT_CASE
+    return nullptr;
+}
+
+// Zero the progress words, the claim counter and the abort word; the last word of
+// the state reads all ones (the dependency of lanes that watch none).
+__global__ void flow_reset(unsigned *__restrict__ w, long n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) w[e] = e == n - 1 ? ~0u : 0u;
+}
+}  // namespace
+
+int flow_grid(int variant, int adjoint, int uniform)
+{
+    // (device, variant, kind) -> resident blocks x CUs, measured once
+    static std::mutex mu;
+    static std::map<long, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    const int kind = adjoint ? 2 : uniform ? 1 : 0;
+    const long key = ((long)dev * 64 + variant) * 4 + kind;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const void *k = flow_kernel_of(variant, adjoint, uniform, 0);
+    if (!k) return 0;
+    const SweepVariant &sv = kVariants[variant];
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, sv.ti * sv.tj * PFAST_CHUNK, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const int g = std::max(1, nb) * std::max(1, cus);
+    cache[key] = g;
+    return g;
+}
+
+hipError_t launch_sweep3d_flow(hipStream_t st, const SweepArgs &a, const FlowArgs &f,
+                               int variant, int adjoint, int uniform)
+{
+    if (variant < 0 || variant >= sweep_variant_count()) return hipErrorInvalidValue;
+    if (f.ntasks == 0) return hipSuccess;
+    const SweepVariant &sv = kVariants[variant];
+    const dim3 grid(f.ntasks);
+    const dim3 block((unsigned)(sv.ti * sv.tj * PFAST_CHUNK));
+    SweepArgs ac = a;
+    FlowArgs fc = f;
+    void *args[] = {&ac, &fc};
+    return hipLaunchKernel(flow_kernel_of(variant, adjoint, uniform, f.o0), grid, block, args, 0,
+                           st);
+}
+
+size_t flow_words(int nchunk, int ntile) { return (size_t)nchunk * 8 * ntile + 4; }
+
+hipError_t launch_flow_reset(hipStream_t st, unsigned *state, size_t words)
+{
+    hipLaunchKernelGGL(flow_reset, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, state,
+                       (long)words);
+    return hipGetLastError();
 }
 
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk)

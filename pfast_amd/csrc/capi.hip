@@ -14,6 +14,7 @@
 
 #include "../../include/pfast.h"
 #include "batch3d.h"
+#include "flow.h"
 #include "context.h"
 #include "pfast_internal.h"
 #include "plain_kernels.h"
@@ -225,6 +226,14 @@ int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable)
 {
     ARGCHK(ctx, "ctx is NULL");
     ctx->tile_skip = enable ? 1 : 0;
+    return PFAST_OK;
+}
+
+int pfast_ctx_set_dataflow(pfast_ctx *ctx, int mode)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ARGCHK(mode >= 0 && mode <= 2, "dataflow mode must be 0, 1 or 2");
+    ctx->dataflow = mode;
     return PFAST_OK;
 }
 
@@ -618,42 +627,79 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
         skip.tmod = s->tmod;
         skip.tclean = s->tclean;
     }
+    const int ftile = ((m->nrows + sv.ti - 1) / sv.ti) * ((m->ncols + sv.tj - 1) / sv.tj);
+    const size_t fwords = pfast::flow_words(s->nchunk, ftile);
+    bool flow_ready = false;  // dataflow state allocated and reset for this phase
+    int epoch = 0;
     while (!act.empty() && cycle < MAXSwp) {
         cycle++;
-        // the active chunks, round-robin over the context's streams (one list each)
-        const int K = std::max(1, std::min(ctx->nstreams, (int)act.size()));
-        std::vector<int> lists;
-        std::vector<int> off(K + 1, 0);
-        for (int k = 0; k < K; k++) {
-            for (size_t i = k; i < act.size(); i += K) lists.push_back(act[i]);
-            off[k + 1] = (int)lists.size();
-        }
-        HIPCHK(hipMemcpyAsync(s->active, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        skip.cycle = cycle - 1;
+        const pfast::TileSkip *sp = skip.tmod ? &skip : nullptr;
         HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
         long real = 0;
         for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
         ctx->units[adjoint ? 1 : 0] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
-        if (K > 1) {
-            HIPCHK(hipEventRecord(ctx->fork_join[0], st));
-            for (int k = 1; k < K; k++) HIPCHK(hipStreamWaitEvent(ctx->aux[k - 1], ctx->fork_join[0], 0));
-        }
-        skip.cycle = cycle - 1;
-        const pfast::TileSkip *sp = skip.tmod ? &skip : nullptr;
-        for (int k = 0; k < K; k++) {
-            hipStream_t sk = k ? ctx->aux[k - 1] : st;
-            const int *lk = s->active + off[k];
-            const int nk = off[k + 1] - off[k];
-            int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp)
-                             : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp);
+        // both schedules update every node in the same order: the choice is per cycle
+        const bool flow = pfast::use_dataflow(ctx, (int)act.size(), ftile, fwords);
+        if (flow) {
+            if (!flow_ready) {
+                if (fwords != s->nflow) {
+                    pfast::dfree(ctx, s->flow);
+                    s->flow = nullptr;
+                    s->nflow = 0;
+                    int rc = pfast::dalloc(ctx, &s->flow, fwords);
+                    if (rc) return rc;
+                    s->nflow = fwords;
+                }
+                HIPCHK(pfast::launch_flow_reset(st, s->flow, fwords));
+                flow_ready = true;
+                epoch = 0;
+            }
+            if (++epoch > 0x7FFF) {  // progress words carry 16 bits of launch epoch (2 per cycle)
+                HIPCHK(pfast::launch_flow_reset(st, s->flow, fwords));
+                epoch = 1;
+            }
+            HIPCHK(hipMemcpyAsync(s->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            int rc = pfast::issue_cycle3d_flow(ctx, g, s->chunks, s->active, (int)act.size(),
+                                               adjoint, s->changed, s->flow, s->nchunk, epoch, st, sp);
             if (rc) return rc;
-        }
-        for (int k = 1; k < K; k++) {
-            HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->aux[k - 1]));
-            HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
+        } else {
+            // the active chunks, round-robin over the context's streams (one list each)
+            const int K = std::max(1, std::min(ctx->nstreams, (int)act.size()));
+            std::vector<int> lists;
+            std::vector<int> off(K + 1, 0);
+            for (int k = 0; k < K; k++) {
+                for (size_t i = k; i < act.size(); i += K) lists.push_back(act[i]);
+                off[k + 1] = (int)lists.size();
+            }
+            HIPCHK(hipMemcpyAsync(s->active, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            if (K > 1) {
+                HIPCHK(hipEventRecord(ctx->fork_join[0], st));
+                for (int k = 1; k < K; k++) HIPCHK(hipStreamWaitEvent(ctx->aux[k - 1], ctx->fork_join[0], 0));
+            }
+            for (int k = 0; k < K; k++) {
+                hipStream_t sk = k ? ctx->aux[k - 1] : st;
+                const int *lk = s->active + off[k];
+                const int nk = off[k + 1] - off[k];
+                int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp)
+                                 : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp);
+                if (rc) return rc;
+            }
+            for (int k = 1; k < K; k++) {
+                HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->aux[k - 1]));
+                HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
+            }
         }
         HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        int aborted = 0;
+        if (flow)
+            HIPCHK(hipMemcpyAsync(&aborted, s->flow + fwords - 2, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         pfast::time_collect(ctx);
+        if (aborted) {
+            pfast::set_error("dataflow sweep: a dependency wait timed out (results void)");
+            return PFAST_ERR_HIP;
+        }
         std::vector<int> next;
         for (int ch : act) {
             const unsigned bits = s->hchanged[ch];
@@ -680,21 +726,20 @@ int diagonals3d(const pfast_ctx *ctx, const Grid3 &g)
     return (g.nrows + sv.ti - 1) / sv.ti + (g.ncols + sv.tj - 1) / sv.tj - 1;
 }
 
-int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
-                      hipStream_t st, const TileSkip *skip)
+namespace {
+// Launch arguments common to the sweep schedules of one grid under ctx's variant.
+SweepArgs sweep_args(const pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks,
+                     unsigned *changed, const TileSkip *skip)
 {
-    if (!st) st = ctx->stream;
     const SweepVariant &sv = sweep_variant(ctx->sweep);
-    const int nti = (g.nrows + sv.ti - 1) / sv.ti, ntj = (g.ncols + sv.tj - 1) / sv.tj;
     SweepArgs a{};
     a.chunks = chunks;
     a.changed = changed;
     a.nrows = g.nrows;
     a.ncols = g.ncols;
     a.nz = g.nz;
-    a.nti = nti;
-    a.ntj = ntj;
+    a.nti = (g.nrows + sv.ti - 1) / sv.ti;
+    a.ntj = (g.ncols + sv.tj - 1) / sv.tj;
     a.hx = (float)g.dx;
     a.hy = (float)g.dy;
     a.hz = (float)g.dz;
@@ -704,6 +749,70 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     const bool skipping = skip && sv.kind == 0;
     a.tmod = skipping ? skip->tmod : nullptr;
     a.tclean = skipping ? skip->tclean : nullptr;
+    return a;
+}
+
+// Wall-clock ticks a dataflow dependency wait may take before the launch gives up
+// (30 s: far beyond any wait of a correct schedule).
+unsigned long long flow_timeout_ticks(int device)
+{
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+        khz <= 0)
+        khz = 100000;
+    return (unsigned long long)khz * 1000ull * 30ull;
+}
+}  // namespace
+
+bool use_dataflow(const pfast_ctx *ctx, int nact, int ntile, size_t words)
+{
+    if (ctx->dataflow == 0 || sweep_variant(ctx->sweep).kind != 0) return false;
+    // the dataflow kernels address a state word by a 26-bit index
+    if (words >= (size_t(1) << 26)) return false;
+    if (ctx->dataflow == 1) return true;
+    return (long)nact * ntile <= kAutoDataflowTasks;
+}
+
+int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *act,
+                       int nact, bool adjoint, unsigned *changed, unsigned *flow, int nids,
+                       int epoch, hipStream_t st, const TileSkip *skip)
+{
+    if (!st) st = ctx->stream;
+    if (nact <= 0) return PFAST_OK;
+    SweepArgs a = sweep_args(ctx, g, chunks, changed, skip);
+    a.active = act;
+    a.nact = nact;
+    a.nact_p8 = nact;
+    const int ntile = a.nti * a.ntj;
+    const size_t nprog = flow_words(nids, ntile) - 4;
+    FlowArgs f{};
+    f.prog = flow;
+    f.nprog = (unsigned)nprog;
+    f.ticket = flow + nprog;
+    f.abort = reinterpret_cast<int *>(flow + nprog + 2);
+    f.ntasks = 4u * (unsigned)ntile * (unsigned)nact;
+    f.cycle = skip ? skip->cycle : 0;
+    f.nd = a.nti + a.ntj - 1;
+    f.timeout = flow_timeout_ticks(ctx->device);
+    for (int half = 0; half < 2; half++) {  // orderings 0-3, then 4-7
+        f.o0 = 4 * half;
+        f.base = (unsigned)(2 * epoch - 1 + half) << pflow::kEpochShift;
+        time_begin(ctx, adjoint ? 1 : 0, st);
+        HIPCHK(launch_sweep3d_flow(st, a, f, ctx->sweep, adjoint ? 1 : 0, g.uniform));
+        time_end(ctx, st);
+    }
+    return PFAST_OK;
+}
+
+int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
+                      hipStream_t st, const TileSkip *skip)
+{
+    if (!st) st = ctx->stream;
+    const SweepVariant &sv = sweep_variant(ctx->sweep);
+    SweepArgs a = sweep_args(ctx, g, chunks, changed, skip);
+    const int nti = a.nti, ntj = a.ntj;
+    const bool skipping = a.tmod != nullptr;
     a.o = o;
     SweepArgs af = a, aa = a;
     af.active = fwd;
@@ -913,7 +1022,7 @@ void shots3d_free(pfast_shots3d *s)
     void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
                     s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
                     s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active, s->chunks,
-                    s->tmod, s->tclean};
+                    s->tmod, s->tclean, s->flow};
     for (void *p : ptrs) pfast::dfree(s->m->ctx, p);
     if (s->hchanged) (void)hipHostFree(s->hchanged);
     delete s;

@@ -41,6 +41,7 @@ struct pfast_ctx {
     // aux[0..nstreams-2]) so that one stream's launch tails fill with another's work
     int nstreams = 1;
     int tile_skip = 1;  // exact tile skipping in the column-window sweeps
+    int dataflow = 2;   // batched 3D cycles as dataflow launches: 0 never, 1 always, 2 auto
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> fork_join;  // [0] fork on stream, [1..] joins on aux
 };

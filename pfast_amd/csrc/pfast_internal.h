@@ -47,6 +47,17 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
                                              0x00020000);
 }
+// A pointer every lane holds the same value of, as a scalar (a buffer descriptor must
+// be; the compiler cannot always tell, e.g. for a pointer read from memory that the
+// kernel also writes, and would otherwise loop over the lanes for every access).
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
 __device__ __forceinline__ float bload(rsrc_t r, unsigned off)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
@@ -66,6 +77,35 @@ __device__ __forceinline__ unsigned bload_u8(rsrc_t r, unsigned off)
 __device__ __forceinline__ void bstore(rsrc_t r, unsigned off, float v)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
+// Device-scope (sc1) buffer access for data that other work-groups of the same
+// launch write or read: on gfx950 each XCD has its own L2, and an access at device
+// scope is coherent across them (the same encoding the compiler gives a relaxed
+// agent-scope atomic).  The batched sweep fields use it throughout; it measured no
+// slower than the default policy on the 400-source workload.
+constexpr int kCpolDevice = 16;  // CPol::SC1
+__device__ __forceinline__ float bload_dev(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kCpolDevice));
+}
+__device__ __forceinline__ float2 bload2_dev(rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kCpolDevice));
+}
+__device__ __forceinline__ void bstore_dev(rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, kCpolDevice);
+}
+template <class T>
+__device__ __forceinline__ T ld_agent(const T *p)
+{
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // vmcnt(0) (expcnt and lgkmcnt left at their maxima).  Issued through the builtin

@@ -101,6 +101,12 @@ struct Group {
     int *tmod = nullptr, *tclean = nullptr;
     int ntile = 0;
     long cycle_no = 0;
+    // dataflow launches (pfast_ctx_set_dataflow): state by chunk id, cycle epochs
+    unsigned *flow = nullptr;
+    size_t nflow = 0;
+    int ftile = 0, fepoch = 0;
+    bool dataflow_cycle = false;      // the cycle in flight runs as dataflow launches
+    unsigned *habort = nullptr;       // pinned: the state's abort word after the cycle
     std::vector<int> reset;  // chunk ids whose records are zeroed before the next cycle
     std::vector<int> cyc_ids;  // chunk ids of the cycle in flight
     std::set<long> cyc_reqs;   // requests of the cycle in flight
@@ -201,6 +207,23 @@ int grow_group(Group &G, int need)
         (void)hipHostFree(G.hact);
         (void)hipHostFree(G.hchanged);
     }
+    unsigned *fl = nullptr;
+    const size_t fw = pfast::flow_words(cap, G.ftile);
+    if (hipMalloc(&fl, fw * sizeof(unsigned)) != hipSuccess ||
+        pfast::launch_flow_reset(G.ctx->stream, fl, fw) != hipSuccess) {
+        (void)hipGetLastError();
+        for (void *q : {(void *)dt, (void *)da, (void *)dc, (void *)tm, (void *)tc, (void *)fl})
+            if (q) (void)hipFree(q);
+        (void)hipHostFree(ht);
+        (void)hipHostFree(ha);
+        (void)hipHostFree(hc);
+        pfast::set_error("allocation of a group's dataflow state failed");
+        return PFAST_ERR_NOMEM;
+    }
+    if (G.flow) (void)hipFree(G.flow);
+    G.flow = fl;
+    G.nflow = fw;
+    G.fepoch = 0;
     if (G.tmod) (void)hipFree(G.tmod);
     if (G.tclean) (void)hipFree(G.tclean);
     G.tmod = tm;
@@ -234,8 +257,8 @@ int group_of(pfast_server *S, const pfast::Grid3 &g, int *out)
     G->ctx->pool = &S->pool;
     {
         const SweepVariant &sv = pfast::sweep_variant(G->ctx->sweep);
-        if (G->ctx->tile_skip && sv.kind == 0)
-            G->ntile = ((g.nrows + sv.ti - 1) / sv.ti) * ((g.ncols + sv.tj - 1) / sv.tj);
+        G->ftile = ((g.nrows + sv.ti - 1) / sv.ti) * ((g.ncols + sv.tj - 1) / sv.tj);
+        if (G->ctx->tile_skip && sv.kind == 0) G->ntile = G->ftile;
     }
     for (auto &e : G->ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
@@ -245,6 +268,13 @@ int group_of(pfast_server *S, const pfast::Grid3 &g, int *out)
             pfast::set_error("hipEventCreate failed");
             return PFAST_ERR_HIP;
         }
+    if (hipHostMalloc((void **)&G->habort, sizeof(unsigned)) != hipSuccess) {
+        for (auto &f : G->ev) (void)hipEventDestroy(f);
+        pfast_ctx_destroy(G->ctx);
+        pfast::set_error("hipHostMalloc failed");
+        return PFAST_ERR_NOMEM;
+    }
+    *G->habort = 0;
     S->groups.push_back(std::move(G));
     *out = (int)S->groups.size() - 1;
     return PFAST_OK;
@@ -408,7 +438,10 @@ int issue_cycle(pfast_server *S, Group &G)
     if (fwd.empty() && adj.empty()) return PFAST_OK;
     for (long id : G.cyc_reqs) S->reqs[id].cycle++;
     if (G.tmod) {
-        if (G.cycle_no > (1L << 20)) {  // keep epochs far below INT_MAX: forget all records
+        // a launch epoch is 1 + ((cycle * 8 + o) * nd + D): keep it below INT_MAX by
+        // forgetting every record before the next cycle would pass it
+        const long nd = pfast::diagonals3d(G.ctx, G.g);
+        if ((G.cycle_no + 2) * 8 * nd >= 0x7fffffffL) {
             HIPCHK(hipMemsetAsync(G.tmod, 0, (size_t)G.cap * G.ntile * 5 * sizeof(int), st));
             HIPCHK(hipMemsetAsync(G.tclean, 0, (size_t)G.cap * G.ntile * 8 * sizeof(int), st));
             G.cycle_no = 0;
@@ -434,6 +467,30 @@ int issue_cycle(pfast_server *S, Group &G)
     G.inflight = true;
     S->cycles++;
     S->node_sweeps += 8.0 * G.g.nrows * G.g.ncols * G.g.nz * (double)real;
+    if (pfast::use_dataflow(G.ctx, na, G.ftile, G.nflow)) {
+        // the whole cycle as dataflow launches (forward chunks, then adjoint chunks);
+        // they share the GPU with other groups' work block by block
+        if (++G.fepoch > 0x7FFF) {
+            HIPCHK(pfast::launch_flow_reset(st, G.flow, G.nflow));
+            G.fepoch = 1;
+        }
+        const pfast::TileSkip skip{G.tmod, G.tclean, (int)G.cycle_no};
+        const pfast::TileSkip *sp = G.tmod ? &skip : nullptr;
+        int rc2 = pfast::issue_cycle3d_flow(G.ctx, G.g, G.dtable, G.dact, G.nfwd, false,
+                                            G.dchanged, G.flow, G.cap, G.fepoch, st, sp);
+        if (!rc2)
+            rc2 = pfast::issue_cycle3d_flow(G.ctx, G.g, G.dtable, G.dact + G.nfwd, G.nadj, true,
+                                            G.dchanged, G.flow, G.cap, G.fepoch, st, sp);
+        if (rc2) return rc2;
+        HIPCHK(hipMemcpyAsync(G.hchanged, G.dchanged, n * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(G.habort, G.flow + G.nflow - 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(G.ev[0], st));
+        G.issued = 1;
+        G.o = 8;
+        G.dataflow_cycle = true;
+        return PFAST_OK;
+    }
+    G.dataflow_cycle = false;
     return pump_cycle(G);
 }
 
@@ -577,12 +634,21 @@ int progress_once(pfast_server *S, int *ndone)
         }
         if (q == hipErrorNotReady) continue;
         G->inflight = false;
-        if (q != hipSuccess) {
+        const bool aborted = q == hipSuccess && G->dataflow_cycle && *G->habort;
+        if (q != hipSuccess || aborted) {
+            const std::string why = aborted ? std::string("a dataflow dependency wait timed out")
+                                            : std::string(hipGetErrorString(q));
             for (long rid : G->cyc_reqs) {
                 SRequest &R = S->reqs[rid];
                 free_chunks(*G, R);
-                fail_request(S, R, std::string("sweep cycle failed: ") + hipGetErrorString(q));
+                fail_request(S, R, "sweep cycle failed: " + why);
                 (*ndone)++;
+            }
+            if (aborted) {  // the state is reset before the group sweeps again
+                *G->habort = 0;
+                if (pfast::launch_flow_reset(G->ctx->stream, G->flow, G->nflow) != hipSuccess)
+                    (void)hipGetLastError();
+                G->fepoch = 0;
             }
             continue;
         }
@@ -716,6 +782,8 @@ int pfast_server_destroy(pfast_server *S)
         }
         if (G->tmod) (void)hipFree(G->tmod);
         if (G->tclean) (void)hipFree(G->tclean);
+        if (G->flow) (void)hipFree(G->flow);
+        if (G->habort) (void)hipHostFree(G->habort);
         for (auto &e : G->ev)
             if (e) (void)hipEventDestroy(e);
         pfast_ctx_destroy(G->ctx);

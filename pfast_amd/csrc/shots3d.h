@@ -32,6 +32,8 @@ struct pfast_shots3d {
     ChunkRef *chunks = nullptr;          // [nchunk] device table the sweep kernels read
     int *tmod = nullptr, *tclean = nullptr;  // tile skip records, [nchunk][ntile](x8)
     int ntile = 0;
+    unsigned *flow = nullptr;            // dataflow state (progress words, counters)
+    size_t nflow = 0;
     std::vector<ChunkRef> hchunks;       // host copy
     // receiver table (reference order) for picks and the single-source path
     long *rnode = nullptr;
@@ -89,6 +91,23 @@ int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const 
 int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
                       int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
                       hipStream_t st = nullptr, const TileSkip *skip = nullptr);
+// Active chunks x tiles of one ordering up to which the automatic schedule takes the
+// dataflow launches: they keep the GPU busy where a launch per tile anti-diagonal
+// has too few tiles (measured on 201x201x101: 2.0x faster with 64 sources, 1.4x with
+// 128, about equal near 20000, and 0.8x with 400 sources, whose diagonals fill the GPU).
+constexpr long kAutoDataflowTasks = 20000;
+// Whether a cycle over nact chunks of a grid of ntile tiles runs as dataflow launches
+// under ctx's schedule setting (0 never, 1 always, 2 automatic); words: the size of
+// the dataflow state.
+bool use_dataflow(const pfast_ctx *ctx, int nact, int ntile, size_t words);
+// One sweep cycle as two dataflow launches (flow.h: orderings 0-3, then 4-7) on
+// stream st over the chunk ids act[0..nact) (device array) of the chunk table,
+// forward or adjoint.  flow: the phase's dataflow state, flow_words(id bound, tiles)
+// words, reset at the phase start; epoch: this cycle's number in the phase (1, 2, ...,
+// at most 0x7FFF); skip as above.
+int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *act,
+                       int nact, bool adjoint, unsigned *changed, unsigned *flow, int nids,
+                       int epoch, hipStream_t st, const TileSkip *skip);
 // Tile anti-diagonals of one ordering under ctx's sweep variant.
 int diagonals3d(const pfast_ctx *ctx, const Grid3 &g);
 
