@@ -60,6 +60,21 @@ int pfast_fsm3d(const double *vp, double *U, const int *src, int srcid, int MAXS
                 double dx, double dy, double dz, int nrows, int ncols, int nz, int pe,
                 int *ncycles);
 
+/* Allocation-free fsm3d.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under their own graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * pfast_fsm3d, bit for bit, but it calls no hipMalloc, hipFree or other allocating
+ * routine: the device copies of vp and U, the convergence copy and the cycle flag live
+ * in a static device array of PFAST_NOALLOC_BYTES that each GPU holds until the process
+ * ends, and the work runs on the null stream.  Grids above PFAST_NOALLOC_MAX_NODES
+ * nodes are refused (PFAST_ERR_ARG).  Calls on one GPU share that array; this library
+ * serializes them with a lock (overlapping calls would otherwise race). */
+#define PFAST_NOALLOC_BYTES (128u << 20)
+#define PFAST_NOALLOC_MAX_NODES 5000000
+int pfast_fsm3d_noalloc(const double *vp, double *U, const int *src, int srcid, int MAXSwp,
+                        double dx, double dy, double dz, int nrows, int ncols, int nz, int pe,
+                        int *ncycles);
+
 /* adjoint_fsm3d, src/T_padjoint3d.h:73.  Writes lambda into U, misfit into *EE. */
 int pfast_adjoint_fsm3d(double *U, const double *T, const double *T_obs, const int *rec,
                         const double *nxyz, const int *src, const int *srnum, int srcid,

@@ -43,6 +43,7 @@ _SIGS = {
     "pfast_version": [],
     "pfast_device_count": [pint],
     "pfast_fsm3d": [dp, dp, ip, i, i, d, d, d, i, i, i, i, vpp],
+    "pfast_fsm3d_noalloc": [dp, dp, ip, i, i, d, d, d, i, i, i, i, vpp],
     "pfast_adjoint_fsm3d": [dp, dp, vpp, vpp, vpp, ip, ip, i, i, d, d, d, i, i, i, i, pdbl, vpp],
     "pfast_rt_fsm2d": [dp, dp, vpp, vpp, vpp, ip, vpp, vpp, i, i, i, d, d, i, i, i, vpp, vpp],
     "pfast_tadjoint_fsm2d": [dp, dp, vpp, d, vpp, vpp, ip, ip, i, i, d, d, i, i, i, pdbl, vpp],

@@ -385,6 +385,92 @@ int pfast_fsm3d(const double *vp, double *U, const int *src, int srcid, int MAXS
     return PFAST_OK;
 }
 
+}  // extern "C"
+
+// ----------------------------------------------- allocation-free forward (3D)
+// Device scratch of pfast_fsm3d_noalloc: the FP64 velocities, traveltimes and the
+// convergence copy of one source plus the cycle flag, in a file-scope array that each
+// GPU the library runs on holds until the process ends.  In host code `scratch` is
+// only a stand-in for the symbol: its device address comes from hipGetSymbolAddress,
+// once per device, and is the only form host code uses.  It has external linkage: the
+// runtime resolves hipGetSymbolAddress by the name in the code object's symbol table,
+// where a `static` array is a local symbol it cannot find (without -fgpu-rdc).
+__device__ __align__(256) unsigned char scratch[PFAST_NOALLOC_BYTES];
+
+namespace {
+
+std::mutex g_noalloc_mu;                 // calls on one device share its scratch
+unsigned char *g_noalloc_ptr[64] = {};   // device address of `scratch`, per device
+
+int noalloc_scratch(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= 64) {
+        pfast::set_error("device index out of range for the allocation-free scratch");
+        return PFAST_ERR_ARG;
+    }
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        // `::scratch`: inside this namespace the unqualified name is the grow-only
+        // scratch allocator above
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || !p) {
+            pfast::set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return PFAST_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return PFAST_OK;
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int pfast_fsm3d_noalloc(const double *vp, double *U, const int *src, int srcid, int MAXSwp,
+                        double dx, double dy, double dz, int nrows, int ncols, int nz, int pe,
+                        int *ncycles)
+{
+    (void)pe;
+    ARGCHK(vp && U && src, "NULL array");
+    ARGCHK(nrows >= 2 && ncols >= 2 && nz >= 2, "grid dimensions must be >= 2");
+    ARGCHK(srcid >= 0, "srcid < 0");
+    const long N = (long)nrows * ncols * nz;
+    if (N > PFAST_NOALLOC_MAX_NODES) {
+        pfast::set_error("grid too large for pfast_fsm3d_noalloc: " + std::to_string(N) +
+                         " nodes, at most " + std::to_string((long)PFAST_NOALLOC_MAX_NODES));
+        return PFAST_ERR_ARG;
+    }
+    const int si = src[srcid * 3], sj = src[srcid * 3 + 1], sk = src[srcid * 3 + 2];
+    ARGCHK(si >= 0 && si < nrows && sj >= 0 && sj < ncols && sk >= 0 && sk < nz,
+           "source node outside the grid");
+    int rc = ensure_device();
+    if (rc) return rc;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char *base = nullptr;
+    if ((rc = noalloc_scratch(dev, &base))) return rc;
+    const size_t nb = align256((size_t)N * sizeof(double));
+    double *dvp = reinterpret_cast<double *>(base);
+    double *dU = reinterpret_cast<double *>(base + nb);
+    double *dp = reinterpret_cast<double *>(base + 2 * nb);
+    int *flag = reinterpret_cast<int *>(base + 3 * nb);
+    // the null stream and no timing: this call creates no context, stream or event
+    pfast_ctx local;
+    local.device = dev;
+    local.stream = nullptr;
+    HIPCHK(hipMemcpyAsync(dvp, vp, N * sizeof(double), hipMemcpyHostToDevice, nullptr));
+    rc = fsm3d_dev(&local, dvp, dU, dp, flag, si, sj, sk, MAXSwp, dx, dy, dz, nrows, ncols, nz,
+                   ncycles);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(U, dU, N * sizeof(double), hipMemcpyDeviceToHost, nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    return PFAST_OK;
+}
+
 int pfast_adjoint_fsm3d(double *U, const double *T, const double *T_obs, const int *rec,
                         const double *nxyz, const int *src, const int *srnum, int srcid,
                         int MAXSwp, double dx, double dy, double dz, int nrows, int ncols, int nz,
