@@ -11,7 +11,12 @@ survey generator, 16 sources x 2000 receivers) through properties that hold at a
   quadruples the misfit;
 - a source's results do not depend on which sources share its chunk: a shot set of
   sources 4..11 (one chunk) reproduces their fields bit for bit from the 16-source set
-  (chunks 0..7 and 8..15).
+  (chunks 0..7 and 8..15);
+- the quantities the precision contract bounds, at this size: the summed gradient
+  g = -sum_s lambda_s / v^3 of all 16 sources within 1e-3 (L2, relative) and the summed
+  misfit within 1e-4 of the same sums over the FP64 form (src/T_PFAST3D.cpp:393-399);
+- the FP64 form itself is bit-identical to the FP64 oracle at this size (traveltimes,
+  lambda, misfit and cycle counts of one source).
 """
 import numpy as np
 import pytest
@@ -92,3 +97,35 @@ def test_fullsize_result_independent_of_chunk_mates(full):
         assert np.array_equal(part.field(k), sh.field(s)), s
         assert np.array_equal(part.field(k, adjoint=True), sh.field(s, adjoint=True)), s
     part.close()
+
+
+def test_fullsize_gradient_and_misfit_against_fp64(full):
+    pf, cfg, m, sh = full
+    g32, E32 = sh.gradient()
+    vp = cfg["vp"]
+    sumL = np.zeros(vp.shape)
+    E64 = 0.0
+    for s in range(NSRC):
+        EE, _, _, L64 = sh.plain_source(s, MAXSwp=100, fields=True)
+        sumL += L64
+        E64 += EE
+    g64 = -sumL / vp ** 3
+    gerr = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    eerr = abs(E32 - E64) / E64
+    print(f"16 sources: gradient {gerr:.2e} (L2), misfit {eerr:.2e}", flush=True)
+    assert gerr <= 1e-3 and eerr <= 1e-4
+
+
+def test_fullsize_plain_fp64_bitexact_vs_oracle(full):
+    import oracle_lib as O
+    pf, cfg, m, sh = full
+    h = cfg["h"]
+    s = 5
+    EE, pc, T64, L64 = sh.plain_source(s, MAXSwp=100, fields=True)
+    To, cf = O.fsm3d(cfg["vp"], cfg["src"], s, h, h, h)
+    Lo, EEo, ca = O.adjoint_fsm3d(To, cfg["T_obs"], cfg["rec"], cfg["nxyz"], cfg["src"],
+                                  cfg["srnum"], s, h, h, h)
+    print(f"source {s}: cycles GPU {pc.tolist()} oracle {[cf, ca]}, EE {EE!r} {EEo!r}", flush=True)
+    assert np.array_equal(T64.view(np.int64), To.view(np.int64))
+    assert np.array_equal(L64.view(np.int64), Lo.view(np.int64))
+    assert EE == EEo and list(pc) == [cf, ca]

@@ -238,3 +238,82 @@ def test_poisson_stream_completes():
     info = srv.info()
     assert info["running"] == 0 and info["queued"] == 0 and info["in_use"] == resident
     srv.close()
+
+
+def test_large_serving_grids_alone_and_in_mix():
+    """Requests on the two largest serving grids (3D 129 x 129 x 65 and 321 x 321 x 161,
+    h = 25 m, pfast_amd.loadgen models): each result is bit-identical alone, in a mix
+    submitted together, and from Shots3D.run."""
+    import pfast_amd
+    from pfast_amd import loadgen as LG
+    grids = [g for g in LG.SERVING_GRIDS if g[0] == "3d" and g[1][0] >= 65 and g[1][1] >= 129]
+    assert [g[1] for g in grids] == [(65, 129, 129), (161, 321, 321)]
+    rng = np.random.default_rng(77)
+    # (grid, kind, sources)
+    plan = [(0, "gradient", 8), (0, "table", 3), (1, "gradient", 1), (1, "table", 2),
+            (1, "gradient", 5)]
+    reqs = [LG.make_request("3d", grids[gi][1], grids[gi][2], kind, n, rng) for gi, kind, n in plan]
+    vps = [LG.velocity_model("3d", g[1], g[2], 40 + i) for i, g in enumerate(grids)]
+
+    def serve(order):
+        srv = pfast_amd.Server(0, 64 << 30)
+        ids = [srv.add_model3d(v, 25.0, 25.0, 25.0) for v in vps]
+        out, tickets = {}, {}
+        for k in order:
+            gi, kind, n = plan[k]
+            r = reqs[k]
+            args = (r["rec"], r["nxyz"], r["T_obs"]) if kind == "gradient" else ()
+            tickets[k] = srv.submit3d(ids[gi], kind, r["src"], r["srnum"], *args)
+            if len(order) == 1:
+                _drain(srv, [tickets[k]])
+        _drain(srv, list(tickets.values()), timeout=300)
+        for k, t in tickets.items():
+            st, _ = srv.status(t)
+            assert st == "done", srv.error(t)
+            n = plan[k][2]
+            res = {"T": np.stack([srv.field(t, s) for s in range(n)]), "cycles": srv.cycles(t)}
+            if plan[k][1] == "gradient":
+                res["grad"] = srv.gradient(t)
+            out[k] = res
+            srv.release(t)
+        srv.close()
+        return out
+
+    alone = {}
+    for k in range(len(plan)):
+        print(f"alone: request {k} {plan[k]}", flush=True)
+        alone.update(serve([k]))
+    print("mix: all requests together", flush=True)
+    mixed = serve([4, 0, 2, 1, 3])
+    for k in range(len(plan)):
+        assert _same(alone[k], mixed[k]), f"request {k} {plan[k]} differs in the mix"
+    ctx = pfast_amd.Context(0)
+    for k in (0, 4):
+        gi, kind, n = plan[k]
+        r = reqs[k]
+        mod = pfast_amd.Model3D(ctx, vps[gi], 25.0, 25.0, 25.0)
+        sh = pfast_amd.Shots3D(mod, r["src"], r["srnum"], r["rec"], r["nxyz"], r["T_obs"])
+        sh.run(adjoint=True)
+        assert _same(np.stack([sh.field(s) for s in range(n)]), alone[k]["T"]), k
+        assert _same(sh.gradient(), alone[k]["grad"]), k
+        sh.close()
+        mod.close()
+    ctx.close()
+
+
+def test_poisson_stream_completes_on_every_grid():
+    """A short Poisson stream over all five serving grids, large ones included."""
+    import pfast_amd
+    from pfast_amd import loadgen as LG
+    srv = pfast_amd.Server(0, 64 << 30)
+    models = LG.load_models(srv, per_grid=1)
+    resident = srv.info()["in_use"]
+    recs = LG.poisson_run(srv, models, 6.0, 3.0, seed=5)
+    res = LG.summarize(recs)
+    print(res, flush=True)
+    assert res["requests"] == len(recs) >= 8
+    assert 0 < res["p50_ms"] <= res["p99_ms"]
+    srv.progress(0)
+    info = srv.info()
+    assert info["running"] == 0 and info["queued"] == 0 and info["in_use"] == resident
+    srv.close()
