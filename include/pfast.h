@@ -268,6 +268,26 @@ double pfast_cubic_interp(double alpha0, double alpha1, double phi0, double phi1
 double pfast_magnitude_of(const double *g, long N);
 double pfast_mean_of(const double *g, long N);
 
+/* Device-resident L-BFGS memory: the LM pairs (s, y) and rho of N doubles that the
+ * drivers keep in sk / yk / rhoi (src/T_PFAST3D.cpp:430-440, src/RT_PFAST2D.cpp:495-505),
+ * on ctx's device, with the two-loop recursion and the line search's dot products run
+ * there in FP64, every dot product a compensated (Neumaier) sum. */
+typedef struct pfast_lbfgs pfast_lbfgs;
+int pfast_lbfgs_create(pfast_ctx *ctx, long N, int LM, pfast_lbfgs **out);
+int pfast_lbfgs_destroy(pfast_lbfgs *h);
+/* L-BFGS direction at iteration Iterk >= 2, replacing gd2sk and the two-loop of
+ * src/T_PFAST3D.cpp:450-484 and src/RT_PFAST2D.cpp:520-545: the pair s = alpha1 dk,
+ * y = gk0 - gk1, rho = 1 / s.y goes to slot (Iterk - 2) % LM, then the recursion runs
+ * over the last min(LM, Iterk - 1) pairs (slot i % LM for pair i: the reference's ai[i]
+ * leaves its array once Iterk >= LM + 2, and i % LM is the evident intent) from
+ * H0 = gamma I, gamma = s.y / y.y.  gk1, gk0: host arrays of N doubles; dk: host array,
+ * in the previous direction, out the new one; *gamma (may be NULL) receives gamma. */
+int pfast_lbfgs_direction(pfast_lbfgs *h, int Iterk, const double *gk1, const double *gk0,
+                          double *dk, double alpha1, double *gamma);
+/* a . b (host arrays of N doubles) on the device, FP64 compensated: the line search's
+ * dk . Dj (src/RT_PFAST2D.cpp:622, src/T_PFAST3D.cpp:520-575). */
+int pfast_lbfgs_dot(pfast_lbfgs *h, const double *a, const double *b, double *out);
+
 /* ---------------------------------------------------------------------------------
  * Request server (no counterpart in the reference: its drivers run one survey per
  * MPI job).  Velocity models stay resident on the device; traveltime-table and

@@ -40,6 +40,10 @@ pint, pdbl, plong = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_lon
 
 # name -> argtypes (all return int status)
 _SIGS = {
+    "pfast_lbfgs_create": [vpp, C.c_long, i, C.POINTER(vpp)],
+    "pfast_lbfgs_destroy": [vpp],
+    "pfast_lbfgs_direction": [vpp, i, dp, dp, dp, d, pdbl],
+    "pfast_lbfgs_dot": [vpp, dp, dp, pdbl],
     "pfast_version": [],
     "pfast_device_count": [pint],
     "pfast_fsm3d": [dp, dp, ip, i, i, d, d, d, i, i, i, i, vpp],

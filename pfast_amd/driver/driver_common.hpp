@@ -4,9 +4,9 @@
 // src/RT_PFAST2D.cpp:412-816 and src/T_PFAST3D.cpp:359-655, whose gradient
 // evaluations here are one batched GPU call instead of a per-source loop.
 //
-// Optimiser arithmetic (dot products, two-loop recursion, clamped update) runs in
-// FP64 on the host in the reference's loop order, so that a driver step depends
-// only on the gradients and misfits it is given.
+// The L-BFGS memory, its two-loop recursion and the line searches' dot products run
+// on the GPU in FP64 with compensated sums (pfast_lbfgs_*); the conjugate-gradient
+// betas and the clamped update stay host loops in the reference's order.
 #pragma once
 #include <cmath>
 #include <cstdio>
@@ -116,9 +116,10 @@ struct Optimiser {
     int schemetag = 3, lsrc = 2, MAXLineSch = 5;
     bool dim2 = true;  // 2D or 3D flavour of the small differences below
     double alpha0 = 0.0, alpha1 = 5.0e3, c1 = 1.0e-4, c2 = 0.9;
-    std::vector<double> Djk1, Djk0, dk, Dj1, sk, yk, rhoi, ai, q, loc_vp;
+    std::vector<double> Djk1, Djk0, dk, Dj1, loc_vp;
+    pfast_lbfgs *lb = nullptr;  // the L-BFGS pairs on the GPU (sk, yk, rhoi of the reference)
 
-    void init(long n, int lm)
+    void init(pfast_ctx *ctx, long n, int lm)
     {
         N = n;
         LM = lm;
@@ -126,18 +127,24 @@ struct Optimiser {
         Djk0.assign(N, 0.0);
         dk.assign(N, 0.0);
         Dj1.assign(N, 0.0);
-        q.assign(N, 0.0);
         loc_vp.assign(N, 0.0);
-        sk.assign((size_t)N * LM, 0.0);
-        yk.assign((size_t)N * LM, 0.0);
-        rhoi.assign(LM, 0.0);
-        ai.assign(LM, 0.0);
+        check(pfast_lbfgs_create(ctx, N, LM, &lb), "pfast_lbfgs_create");
+    }
+
+    ~Optimiser() { pfast_lbfgs_destroy(lb); }
+
+    // FP64 compensated a . b on the GPU
+    double dot(const std::vector<double> &a, const std::vector<double> &b) const
+    {
+        double d = 0.0;
+        check(pfast_lbfgs_dot(lb, a.data(), b.data(), &d), "pfast_lbfgs_dot");
+        return d;
     }
 
     // Search direction at iteration Iterk (src/RT_PFAST2D.cpp:500-552).  Returns
-    // false for an unknown scheme (the reference aborts).  The L-BFGS first loop
-    // reads ai[i % LM]: the reference reads ai[i], which leaves the array once
-    // Iterk >= LM + 2 (src/RT_PFAST2D.cpp:529, src/T_PFAST3D.cpp:467).
+    // false for an unknown scheme (the reference aborts).  The L-BFGS recursion
+    // reads pair i from slot i % LM: the reference reads ai[i], which leaves the
+    // array once Iterk >= LM + 2 (src/RT_PFAST2D.cpp:529, src/T_PFAST3D.cpp:467).
     bool direction(int Iterk)
     {
         if (Iterk == 1) {
@@ -156,24 +163,9 @@ struct Optimiser {
             return true;
         }
         case 3: {
-            for (long j = 0; j < N; j++) q[j] = Djk1[j];
-            const double gamma = pfast_gd2sk(Djk1.data(), Djk0.data(), dk.data(), alpha1,
-                                             rhoi.data(), sk.data(), yk.data(), Iterk, LM, N);
-            const int lo = (Iterk - LM - 1 >= 0) ? Iterk - LM - 1 : 0;
-            for (int i = Iterk - 2; i >= lo; i--) {
-                const int m = i % LM;
-                double a = 0.0;
-                for (long j = 0; j < N; j++) a += rhoi[m] * sk[(size_t)m * N + j] * q[j];
-                ai[m] = a;
-                for (long j = 0; j < N; j++) q[j] -= ai[m] * yk[(size_t)m * N + j];
-            }
-            for (long j = 0; j < N; j++) dk[j] = gamma * q[j];
-            for (int i = lo; i <= Iterk - 2; i++) {
-                const int m = i % LM;
-                double tmp = 0.0;
-                for (long j = 0; j < N; j++) tmp += rhoi[m] * yk[(size_t)m * N + j] * dk[j];
-                for (long j = 0; j < N; j++) dk[j] += sk[(size_t)m * N + j] * (ai[m] - tmp);
-            }
+            check(pfast_lbfgs_direction(lb, Iterk, Djk1.data(), Djk0.data(), dk.data(), alpha1,
+                                        nullptr),
+                  "pfast_lbfgs_direction");
             alpha0 = 0.0;
             alpha1 = 1.0;
             c1 = 1.0e-4;
@@ -186,12 +178,7 @@ struct Optimiser {
     }
 
     // dphi = -dk . g (src/RT_PFAST2D.cpp:622)
-    double dphi(const std::vector<double> &g) const
-    {
-        double d = 0.0;
-        for (long i = 0; i < N; i++) d += -dk[i] * g[i];
-        return d;
-    }
+    double dphi(const std::vector<double> &g) const { return -dot(dk, g); }
 
     // Line search from vp along dk (src/RT_PFAST2D.cpp:561-792); phi0 is the misfit
     // of the current model.  On return alpha1 is the step.  `eval` regularises
@@ -213,8 +200,7 @@ struct Optimiser {
                 // the reference zeroes f0 right before the update (its running f0 of
                 // the previous trial is never used)
                 f0 = 0.0;
-                f1 = 0.0;
-                for (long i = 0; i < N; i++) f1 = f1 + Dj1[i] * dk[i];
+                f1 = dot(Dj1, dk);
                 const double alphak = alpha1 - (alpha1 - alpha0) * f1 / (f1 - f0);
                 alpha0 = alpha1;
                 alpha1 = alphak;

@@ -215,7 +215,7 @@ int main(int argc, char **argv)
           "pfast_shots2d_create");
     const int MAXSwp = 500;
     drv::Optimiser opt;
-    opt.init(N, LM);
+    opt.init(ctx, N, LM);
     opt.schemetag = schemetag;
     opt.lsrc = lsrc;
     opt.MAXLineSch = 5;

@@ -149,7 +149,7 @@ int main(int argc, char **argv)
     }
 
     drv::Optimiser opt;
-    opt.init(N, LM);
+    opt.init(ctx, N, LM);
     opt.schemetag = 3;  // the reference fixes L-BFGS (src/T_PFAST3D.cpp:428)
     opt.lsrc = lsrc;
     opt.MAXLineSch = 2;
