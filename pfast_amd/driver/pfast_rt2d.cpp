@@ -5,11 +5,14 @@
 // source loops, and the gradient smoothing on the GPU (pfast_regu2d).
 //
 // usage: pfast_rt2d input.inp [LM=5 [RTtag=3]] [--forward] [--no-shot-gather]
-//                   [--max-iter K]
+//                   [--max-iter K] [--vmin V] [--vmax V]
 //   --forward         forward modelling only (the reference's -DNoInverse build):
 //                     writes <prefix>_Tr2D.bin and the per-source TTd / TTu fields
 //   --no-shot-gather  with --forward, skip the per-source field files
 //   --max-iter K      stop after K iterations (default 500, the reference's cap)
+//   --vmin V, --vmax V  the velocity clamp of each model update (default the
+//                     reference's MINV / MAXV = 1600 / 4500, src/RT_padjoint2d.h:17,21;
+//                     a model slower than 1600 m/s is lifted to it by the first update)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,7 +36,7 @@ struct Survey2D {
 void usage()
 {
     printf(" usage: pfast_rt2d input_file [Maximum Memory Number=5] [Tomography Scheme=3, Joint]"
-           " [--forward] [--no-shot-gather] [--max-iter K]\n");
+           " [--forward] [--no-shot-gather] [--max-iter K] [--vmin V] [--vmax V]\n");
 }
 
 }  // namespace
@@ -43,8 +46,11 @@ int main(int argc, char **argv)
     std::vector<const char *> pos;
     bool forward = false, shot_gather = true;
     int MAXIter = 500;
+    double vmin = 1600, vmax = 4500;
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "--forward")) forward = true;
+        else if (!strcmp(argv[a], "--vmin") && a + 1 < argc) vmin = atof(argv[++a]);
+        else if (!strcmp(argv[a], "--vmax") && a + 1 < argc) vmax = atof(argv[++a]);
         else if (!strcmp(argv[a], "--no-shot-gather")) shot_gather = false;
         else if (!strcmp(argv[a], "--max-iter") && a + 1 < argc) MAXIter = atoi(argv[++a]);
         else pos.push_back(argv[a]);
@@ -282,7 +288,7 @@ int main(int argc, char **argv)
             return 1;
         }
         opt.line_search(vp, global_E[Iterk - 1], trial);
-        drv::updatevp(vp, opt.dk, opt.alpha1, 1600, 4500);
+        drv::updatevp(vp, opt.dk, opt.alpha1, vmin, vmax);
         drv::writemod(vp.data(), p_vpfile + "_" + k, N);
         convflag = drv::check_conv(opt.dk, opt.alpha1, vp, global_E, Iterk, 0.001, N);
         printf("--------Interating No. %d done---------------\n\n", Iterk);

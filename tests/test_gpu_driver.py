@@ -310,3 +310,69 @@ def test_rt2d_example_inversion_matches_reference_logs(tmp_path, tag):
     assert _rel(t[1, 1], rt[1, 1]) <= 1e-2
     assert _rel(steps[0], rstep[0]) <= 1e-2
     assert np.all(np.abs(res[1] - rres[1]) <= 1e-3 * rres[1])
+
+
+def test_rt2d_config2_tomography_lm5_through_the_ring_wrap(tmp_path):
+    """The 2D first-arrival tomography workload (101 x 401, h = 10 m, 60 surface sources x
+    200 receivers, picks = the true model's FP64 traveltimes + N(0, 2 ms), starting model
+    the 1D gradient; pfast_amd.synth.config2d(2)) through the drop-in driver with L-BFGS
+    memory 5 for 10 iterations (RTtag 1, transmission only): iterations 7..10 use the ring
+    past the point where the reference's own ai[i] leaves its array.  Conditional parity
+    at every model the driver writes: the FP64 oracle's misfit equals the driver's logged
+    misfit within 1e-4.  The final model must be closer to the true model than the start."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle_lib as O
+    import pfast_amd
+    from pfast_amd import formats as F
+    from pfast_amd import synth
+    cfg = synth.config2d(2)
+    h, nr, nc = cfg["h"], cfg["nrows"], cfg["ncols"]
+    nsrc, nrec = cfg["src"].shape[0], int(cfg["srnum"][0])
+    rng = np.random.default_rng(20261204)
+    tt = np.empty(nsrc * nrec)
+    for s in range(nsrc):
+        T = pfast_amd.RT_fsm2d(cfg["vp"], None, cfg["src"], s, 0, 500, h, h)["T"]
+        rows = slice(s * nrec, (s + 1) * nrec)
+        tt[rows] = T[cfg["rec"][rows, 0], cfg["rec"][rows, 1]]
+    tt += rng.normal(0.0, 0.002, tt.size)
+    d = str(tmp_path)
+    rec = np.concatenate([cfg["rec_xz"], cfg["nxy"], tt[:, None]], 1)
+    F.write_bin(os.path.join(d, "m.vp"), cfg["vp0"])
+    F.write_bin(os.path.join(d, "m.src"), cfg["src_tab"])
+    F.write_bin(os.path.join(d, "m.rec"), rec)
+    F.write_bin(os.path.join(d, "m.ref"), np.zeros((0, 3)))
+    F.write_bin(os.path.join(d, "m.wr"), np.zeros((nr, nc)))
+    F.write_inp2d(os.path.join(d, "m.inp"), "m.vp", "m.src", cfg["src_tab"].shape, "m.rec",
+                  rec.shape, "m.ref", (0, 3), nr, nc, h, h, vpfile="final.vp", nux=60, nuz=20,
+                  twfile="m.wr", schemetag=3, lsrc=2)
+    # the model starts at 1500 m/s: the reference's 1600 m/s clamp would lift its top rows
+    # at the first update (src/RT_padjoint2d.h:17), so the clamp is lowered to 1400
+    log = _run("pfast_rt2d", d, ["5", "1", "--max-iter", "10", "--no-shot-gather", "--vmin",
+                                 "1400"])
+    E = np.loadtxt(os.path.join(d, "m_GlobalErr_TR_T_R.txt"), ndmin=2)
+    print("misfits", E[:, 0].tolist(), flush=True)
+    print("\n".join(ln for ln in log.split("\n") if "Stepsize" in ln), flush=True)
+    nmod = len(E) - 1
+    assert nmod >= 7, "the run stopped before the ring wrapped"
+    S = F.load_survey2d(os.path.join(d, "m.inp"))
+
+    def oracle_misfit(vp):
+        def one(s):
+            o = O.rt_fsm2d(vp, None, S["src"], s, 0, h, h)
+            return O.tadjoint_fsm2d(o["T"], S["TT_obs"], 1.0, S["rec"], S["nxy"], S["src"],
+                                    S["srnum"], s, h, h)[1]
+        with ThreadPoolExecutor(8) as ex:
+            return sum(ex.map(one, range(nsrc)))
+
+    for k in range(1, nmod + 1):
+        vp = F.read_bin(os.path.join(d, f"m_{k}"), (nr, nc))
+        Eo = oracle_misfit(vp)
+        print(f"model after iteration {k}: oracle E {Eo:.8g}, driver {E[k, 0]:.8g}", flush=True)
+        assert _rel(E[k, 0], Eo) <= 1e-4
+    final = F.read_bin(os.path.join(d, f"m_{nmod}"), (nr, nc))
+    e0 = np.linalg.norm(cfg["vp0"] - cfg["vp"]) / np.linalg.norm(cfg["vp"])
+    e1 = np.linalg.norm(final - cfg["vp"]) / np.linalg.norm(cfg["vp"])
+    print(f"model error vs the true model: start {e0:.4e}, after {nmod} iterations {e1:.4e}; "
+          f"misfit {E[0, 0]:.6g} -> {E[-1, 0]:.6g}", flush=True)
+    assert e1 < e0 and E[-1, 0] < E[0, 0]
