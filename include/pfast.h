@@ -169,12 +169,19 @@ int pfast_shots3d_plain_source(pfast_shots3d *s, int srcid, int MAXSwp, double *
 
 /* Profiling: accumulated device time (ms) and launch count of the dominant sweep
  * kernels of this context since the last reset, measured with HIP events on the
- * context's stream.  kind 0 = forward sweeps, 1 = adjoint sweeps (3D), 2 = 2D
- * shot-set solves (forward and adjoints). */
+ * context's stream.  kind 0 / 1 = 3D forward / adjoint sweeps launched per tile
+ * anti-diagonal (fwd3d_col / adj3d_col), 2 = 2D shot-set solves (forward and
+ * adjoints), 3 / 4 = 3D forward / adjoint sweeps as dataflow launches (fwd3d_flow /
+ * adj3d_flow). */
 int pfast_ctx_timing(pfast_ctx *ctx, int kind, double *ms, long *launches);
 /* Node-sweeps (grid nodes x real sources x orderings) issued by the batched
- * sweeps of `kind` since the last timing reset: the work the timed launches did. */
+ * sweeps of `kind` since the last timing reset: every node of every active chunk,
+ * the sweep the reference performs. */
 int pfast_ctx_work(pfast_ctx *ctx, int kind, double *node_sweeps);
+/* The node-sweeps of the tiles the timed launches of `kind` actually swept (tiles
+ * left out by exact tile skipping excluded), counted on the device while timing is
+ * enabled. */
+int pfast_ctx_work_swept(pfast_ctx *ctx, int kind, double *node_sweeps);
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable);
 int pfast_ctx_timing_reset(pfast_ctx *ctx);
 /* Batched sweep kernel variant of this context (0 = default).  Every variant

@@ -187,12 +187,18 @@ class Context:
         """Split batched 3D cycles over n streams (1..4); results do not change."""
         check(lib().pfast_ctx_set_streams(self.h, int(n)))
 
+    # timed kinds (pfast_ctx_timing)
+    FWD_DIAG, ADJ_DIAG, SOLVE_2D, FWD_FLOW, ADJ_FLOW = 0, 1, 2, 3, 4
+
     def timing_read(self, kind):
-        """(device ms, launches, node-sweeps) of sweep kind 0 (forward) / 1 (adjoint)."""
-        ms, n, w = C.c_double(0), C.c_long(0), C.c_double(0)
+        """(device ms, launches, node-sweeps issued, node-sweeps swept) of a timed kind:
+        FWD_DIAG / ADJ_DIAG (per-diagonal launches), FWD_FLOW / ADJ_FLOW (dataflow
+        launches), SOLVE_2D."""
+        ms, n, w, sw = C.c_double(0), C.c_long(0), C.c_double(0), C.c_double(0)
         check(lib().pfast_ctx_timing(self.h, int(kind), C.byref(ms), C.byref(n)))
         check(lib().pfast_ctx_work(self.h, int(kind), C.byref(w)))
-        return ms.value, n.value, w.value
+        check(lib().pfast_ctx_work_swept(self.h, int(kind), C.byref(sw)))
+        return ms.value, n.value, w.value, sw.value
 
 
 def sweep_variants():

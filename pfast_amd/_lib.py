@@ -76,6 +76,7 @@ _SIGS = {
     "pfast_shots3d_cycles": [vpp, ip],
     "pfast_shots3d_plain_source": [vpp, i, i, pdbl, vpp, vpp, vpp],
     "pfast_ctx_work": [vpp, i, pdbl],
+    "pfast_ctx_work_swept": [vpp, i, pdbl],
     "pfast_model2d_create": [vpp, dp, i, i, d, d, vpp, i, C.POINTER(vpp)],
     "pfast_model2d_set_vp": [vpp, dp],
     "pfast_model2d_destroy": [vpp],

@@ -35,6 +35,9 @@ struct SweepArgs {
     // change nothing: skipped.
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
+    // node-sweeps of the tiles actually swept (columns x depth x real sources), added up
+    // on the device while profiling (NULL: not counted)
+    unsigned long long *swept;
 };
 
 // State of a dataflow sweep launch (the four orderings of one depth direction of a

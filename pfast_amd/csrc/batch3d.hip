@@ -195,6 +195,16 @@ __device__ __forceinline__ unsigned col_faces(bool valid, int li, int lj, int TI
            (li == TIe - 1 ? 8u : 0u);
 }
 
+// Profiling: the node-sweeps this tile sweep performed (columns x depth x the chunk's
+// real sources; padding sources have node -1).
+__device__ __forceinline__ void count_swept(const SweepArgs &a, const TileCtx &t)
+{
+    if (!a.swept || threadIdx.x != 0) return;
+    int nreal = 0;
+    for (int c = 0; c < PFAST_CHUNK; c++) nreal += a.chunks[t.chunk].srcnode[c] >= 0 ? 1 : 0;
+    atomicAdd(a.swept, (unsigned long long)t.TIe * t.TJe * a.nz * nreal);
+}
+
 // Record the outcome of this tile sweep: `what` = bit 4 (some node changed) | the
 // faces on which a node changed, OR-ed over the block.
 __device__ __forceinline__ void record_tile(const SweepArgs &a, int ix, unsigned what, int *any)
@@ -498,6 +508,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+    count_swept(a, t);
     record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
                 any_change);
 }
@@ -679,6 +690,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
+    count_swept(a, t);
     record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
                 any_change);
 }

@@ -161,6 +161,7 @@ int pfast_ctx_destroy(pfast_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->cache)
         if (e.second.first) (void)hipFree(e.second.first);
+    if (ctx->dswept) (void)hipFree(ctx->dswept);
     for (auto e : ctx->evpool) (void)hipEventDestroy(e);
     for (auto e : ctx->fork_join) (void)hipEventDestroy(e);
     for (auto a : ctx->aux) {
@@ -240,6 +241,13 @@ int pfast_ctx_set_dataflow(pfast_ctx *ctx, int mode)
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable)
 {
     ARGCHK(ctx, "ctx is NULL");
+    if (enable && !ctx->dswept) {
+        HIPCHK(hipSetDevice(ctx->device));
+        int rc = pfast::dalloc(&ctx->dswept, pfast::kTimedKinds);
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(ctx->dswept, 0, pfast::kTimedKinds * sizeof(unsigned long long),
+                              ctx->stream));
+    }
     ctx->timing = enable != 0;
     return PFAST_OK;
 }
@@ -255,6 +263,25 @@ int pfast_ctx_timing_reset(pfast_ctx *ctx)
         ctx->launches[k] = 0;
         ctx->units[k] = 0;
     }
+    if (ctx->dswept) {
+        HIPCHK(hipMemsetAsync(ctx->dswept, 0, pfast::kTimedKinds * sizeof(unsigned long long),
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return PFAST_OK;
+}
+
+int pfast_ctx_work_swept(pfast_ctx *ctx, int kind, double *node_sweeps)
+{
+    ARGCHK(ctx && kind >= 0 && kind < pfast::kTimedKinds && node_sweeps, "bad argument");
+    *node_sweeps = 0;
+    if (!ctx->dswept) return PFAST_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto a : ctx->aux) HIPCHK(hipStreamSynchronize(a));
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpy(&v, ctx->dswept + kind, sizeof(v), hipMemcpyDeviceToHost));
+    *node_sweeps = (double)v;
     return PFAST_OK;
 }
 
@@ -724,9 +751,11 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
         HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
         long real = 0;
         for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
-        ctx->units[adjoint ? 1 : 0] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
         // both schedules update every node in the same order: the choice is per cycle
         const bool flow = pfast::use_dataflow(ctx, (int)act.size(), ftile, fwords);
+        const int kind = flow ? (adjoint ? pfast::kAdjFlow : pfast::kFwdFlow)
+                              : (adjoint ? pfast::kAdjDiag : pfast::kFwdDiag);
+        ctx->units[kind] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
         if (flow) {
             if (!flow_ready) {
                 if (fwords != s->nflow) {
@@ -866,6 +895,7 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
     if (!st) st = ctx->stream;
     if (nact <= 0) return PFAST_OK;
     SweepArgs a = sweep_args(ctx, g, chunks, changed, skip);
+    a.swept = ctx->timing && ctx->dswept ? ctx->dswept + (adjoint ? kAdjFlow : kFwdFlow) : nullptr;
     a.active = act;
     a.nact = nact;
     a.nact_p8 = nact;
@@ -883,7 +913,7 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
     for (int half = 0; half < 2; half++) {  // orderings 0-3, then 4-7
         f.o0 = 4 * half;
         f.base = (unsigned)(2 * epoch - 1 + half) << pflow::kEpochShift;
-        time_begin(ctx, adjoint ? 1 : 0, st);
+        time_begin(ctx, adjoint ? kAdjFlow : kFwdFlow, st);
         HIPCHK(launch_sweep3d_flow(st, a, f, ctx->sweep, adjoint ? 1 : 0, g.uniform));
         time_end(ctx, st);
     }
@@ -901,6 +931,10 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     const bool skipping = a.tmod != nullptr;
     a.o = o;
     SweepArgs af = a, aa = a;
+    if (ctx->timing && ctx->dswept) {
+        af.swept = ctx->dswept + kFwdDiag;
+        aa.swept = ctx->dswept + kAdjDiag;
+    }
     af.active = fwd;
     af.nact = nfwd;
     af.nact_p8 = (nfwd + 7) / 8 * 8;
@@ -913,12 +947,12 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
         const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
         if (skipping) af.epoch = aa.epoch = 1 + ((skip->cycle * 8 + o) * nd + D);
         if (nfwd) {
-            time_begin(ctx, 0, st);
+            time_begin(ctx, kFwdDiag, st);
             HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
             time_end(ctx, st);
         }
         if (nadj) {
-            time_begin(ctx, 1, st);
+            time_begin(ctx, kAdjDiag, st);
             HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
             time_end(ctx, st);
         }

@@ -25,9 +25,13 @@ struct pfast_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool timing = false;
-    double ms[3] = {0, 0, 0};     // per timed kind: forward sweeps, adjoint sweeps, 2D solves
-    long launches[3] = {0, 0, 0};
-    double units[3] = {0, 0, 0};  // node-sweeps of real sources issued (batched paths)
+    // per timed kind (pfast::TimedKind): device ms, launches, node-sweeps of real sources
+    // issued (every node of every active chunk) and swept (tiles that skipping left out
+    // excluded; counted on the device while timing is on)
+    double ms[5] = {0, 0, 0, 0, 0};
+    long launches[5] = {0, 0, 0, 0, 0};
+    double units[5] = {0, 0, 0, 0, 0};
+    unsigned long long *dswept = nullptr;  // device counters, one per kind
     int sweep = 0;                // batched sweep kernel variant
     int sweep2d = 0;              // batched 2D sweep form (0 row window, 1 plane loop)
     int nmodels = 0;              // live models bound to this context
@@ -48,7 +52,10 @@ struct pfast_ctx {
 
 namespace pfast {
 
-constexpr int kTimedKinds = 3;
+// Timed kinds: batched 3D forward / adjoint sweeps launched per tile anti-diagonal,
+// 2D shot-set solves, batched 3D forward / adjoint sweeps as dataflow launches.
+enum TimedKind { kFwdDiag = 0, kAdjDiag = 1, k2D = 2, kFwdFlow = 3, kAdjFlow = 4 };
+constexpr int kTimedKinds = 5;
 
 // An event pair around one launch when timing is enabled; folded into the
 // totals by time_collect after a stream synchronisation.
