@@ -138,6 +138,16 @@ int pfast_shots3d_destroy(pfast_shots3d *s);
  * gradient/misfit reductions) for every source of the set.  MAXSwp caps the sweep
  * cycles per solve.  Results stay on the device until fetched. */
 int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp);
+/* Source neighbourhood (no counterpart in the reference, which seeds T = 0 at the
+ * source node alone, src/T_pfsm3d.h:139-148; that is the default, halfwidth 0, and
+ * the parity mode).  With halfwidth h in 1..PFAST_SRCINIT_MAX, every later forward
+ * of the set first sets the nodes within h nodes of each source along every axis
+ * (a box clipped to the grid) to FP64 traveltimes of the linear velocity model
+ * fitted at the source node -- exact for a model linear in depth -- and the sweeps
+ * never change them: the first-order error of the source singularity no longer
+ * builds up near the source.  Takes effect at the next pfast_shots3d_run. */
+#define PFAST_SRCINIT_MAX 16
+int pfast_shots3d_set_source_init(pfast_shots3d *s, int halfwidth);
 /* Sweep-variant tuning (no counterpart in the reference, whose sweeps have one fixed
  * loop order): times two forward cycles of this shot set under each listed variant
  * (all when variants is NULL), and two adjoint cycles after them when the set has

@@ -273,6 +273,12 @@ class Shots3D:
         check(lib().pfast_shots3d_run(self.h, PFAST_FWD_ADJ if adjoint else PFAST_FWD_ONLY,
                                       int(MAXSwp)))
 
+    def set_source_init(self, halfwidth):
+        """Fix the traveltimes of the nodes within `halfwidth` nodes of each source to
+        FP64 values of the linear velocity model fitted at the source (0, the default:
+        the source node alone, as the reference seeds it).  Applies from the next run."""
+        check(lib().pfast_shots3d_set_source_init(self.h, int(halfwidth)))
+
     def autotune(self, variants=None):
         """Time two forward cycles (and two adjoint cycles, when the shot set has picks)
         of this shot set under each sweep variant (all of them by default), set the

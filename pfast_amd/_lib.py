@@ -69,6 +69,7 @@ _SIGS = {
     "pfast_shots3d_create": [vpp, i, ip, ip, vpp, vpp, vpp, C.POINTER(vpp)],
     "pfast_shots3d_destroy": [vpp],
     "pfast_shots3d_run": [vpp, i, i],
+    "pfast_shots3d_set_source_init": [vpp, i],
     "pfast_shots3d_autotune": [vpp, vpp, i, pint, vpp],
     "pfast_shots3d_gradient": [vpp, vpp, pdbl],
     "pfast_shots3d_picks": [vpp, dp],

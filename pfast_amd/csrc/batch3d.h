@@ -16,6 +16,9 @@ struct ChunkRef {
     const float *slow8;        // the same values 8x8-tile-major: [nz][nti8][ntj8][8][8]
                                // (tile (ti,tj) row li column lj; edge tiles padded)
     int srcnode[PFAST_CHUNK];  // linear source node, -1 for padding sources
+    int srcinit;               // half-width in nodes of the box around each source whose
+                               // traveltimes are set at the start and never swept (0:
+                               // the source node alone, as the reference seeds it)
 };
 
 struct SweepArgs {
@@ -79,6 +82,9 @@ size_t flow_words(int nchunk, int ntile);
 // captured graph.
 hipError_t launch_flow_reset(hipStream_t st, unsigned *state, size_t words);
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
+hipError_t launch_init_src_box(hipStream_t st, float *F, const double *vp, const int *srcnode,
+                               long N, int nchunk, int R, int nrows, int ncols, int nz,
+                               double dx, double dy, double dz);
 hipError_t launch_adj_mark(hipStream_t st, float *T, const long *pnode, const int *psrc,
                            int npick, long N, int unmark);
 // The adjoint field LT interleaves (lambda, T) per node and source: 2 floats per

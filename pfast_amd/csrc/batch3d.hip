@@ -360,15 +360,22 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
         pub[2 * NT] = pub[PB + 2 * NT] = INF;
         *any_change = 0;
     }
-    // steps p at which this thread updates its node: pipj <= p < pipj + nz, except
-    // the level of its own source
-    const int plo = l.valid ? l.pipj : 0x3fffffff, phi = l.pipj + nz;
-    int psrc = -1;
+    // steps p at which this thread updates its node: plo = pipj <= p < pipj + nz, except
+    // the steps pfl .. pfl + pfn at which its column crosses its source's fixed box
+    // (the source node alone when the box half-width R is 0; levels of the box outside
+    // the grid fall outside the column's steps, so the range needs no clipping)
+    const int plo = l.valid ? l.pipj : 0x3fffffff;
+    const int R = a.chunks[t.chunk].srcinit;
+    const unsigned pfn = 2u * (unsigned)R;
+    int pfl = 0x3fffffff;
     {
         const int sn = a.chunks[t.chunk].srcnode[l.c];
-        if (l.valid && sn >= 0 && (unsigned)sn % plane == l.node) {
+        if (l.valid && sn >= 0) {
+            const unsigned sc = (unsigned)sn % plane;
+            const int si = (int)(sc / (unsigned)a.ncols), sj = (int)(sc % (unsigned)a.ncols);
             const int sk = (int)((unsigned)sn / plane);
-            psrc = l.pipj + (s3 > 0 ? sk : nz - 1 - sk);
+            if (abs(t.i0 + l.li - si) <= R && abs(t.j0 + l.lj - sj) <= R)
+                pfl = l.pipj + (s3 > 0 ? sk - R : nz - 1 - sk - R);
         }
     }
     const unsigned st = l.step;
@@ -474,7 +481,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             const float fval = quad_bcast(q[3][u >> 2], u);
             float nv = vcur;
             bool c = false;
-            if (p >= plo && p < phi && p != psrc) {
+            if ((unsigned)(p - plo) < (unsigned)nz && (unsigned)(p - pfl) > pfn) {
                 float qv;
                 if (UNI) qv = eik3d_f32_uniform(xa, ya, za, fval);
                 else qv = eik3d_f32_general(xa, ya, za, fval, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
@@ -909,6 +916,49 @@ __global__ void init_fwd(float *__restrict__ F, const int *__restrict__ srcnode,
         F[e * C + c] = srcnode[chunk * C + c] == n ? 0.0f : PFAST_INF_F;
 }
 
+// Fixed box around each source (half-width R > 0 nodes, clipped to the grid): FP64
+// traveltimes of the linear velocity model v(x) = v_s + G.(x - x_s) fitted at the
+// source node (G from centred differences of vp, one-sided at a grid edge), whose
+// rays are circular arcs with the closed-form traveltime
+//   T = acosh(1 + |G|^2 d^2 / (2 v_s v(x))) / |G|,   d = |x - x_s|,
+// evaluated as log1p(u + sqrt(u (u + 2))) / |G| (no cancellation for small u), and
+// d / sqrt(v_s v(x)) = d / v_s where G = 0.  Exact for a model linear in depth.  One
+// block per (chunk, source) slot; padding sources (node -1) are left alone.
+template <int C>
+__global__ void init_src_box(float *__restrict__ F, const double *__restrict__ vp,
+                             const int *__restrict__ srcnode, long N, int R, int nrows,
+                             int ncols, int nz, double dx, double dy, double dz)
+{
+    const int slot = blockIdx.x;
+    const int sn = srcnode[slot];
+    if (sn < 0) return;
+    const long plane = (long)nrows * ncols;
+    const int sk = (int)(sn / plane), sc = (int)(sn % plane);
+    const int si = sc / ncols, sj = sc % ncols;
+    const double vs = vp[sn];
+    auto slope = [&](int n, int pos, long stride, double h) {
+        if (n < 2) return 0.0;
+        const int lo = pos > 0 ? pos - 1 : pos, hi = pos < n - 1 ? pos + 1 : pos;
+        return (vp[sn + (hi - pos) * stride] - vp[sn + (lo - pos) * stride]) / ((hi - lo) * h);
+    };
+    const double gx = slope(ncols, sj, 1, dx), gy = slope(nrows, si, ncols, dy),
+                 gz = slope(nz, sk, plane, dz);
+    const double g2 = gx * gx + gy * gy + gz * gz, g = sqrt(g2);
+    const int W = 2 * R + 1;
+    float *Fs = F + (long)(slot / C) * N * C + (slot % C);
+    for (int e = threadIdx.x; e < W * W * W; e += blockDim.x) {
+        const int dj = e % W - R, di = (e / W) % W - R, dk = e / (W * W) - R;
+        const int j = sj + dj, i = si + di, k = sk + dk;
+        if (j < 0 || j >= ncols || i < 0 || i >= nrows || k < 0 || k >= nz) continue;
+        const double X = dj * dx, Y = di * dy, Z = dk * dz;
+        const double d2 = X * X + Y * Y + Z * Z;
+        const double vr = fmax(vs + gx * X + gy * Y + gz * Z, 0.1 * vs);
+        const double u = g2 * d2 / (2.0 * vs * vr);
+        const double T = u > 1e-30 ? log1p(u + sqrt(u * (u + 2.0))) / g : sqrt(d2 / (vs * vr));
+        Fs[((long)k * plane + (long)i * ncols + j) * C] = (float)T;
+    }
+}
+
 // Mark receiver nodes of each source as fixed (negative traveltime) and form the
 // per-source misfit terms.  One thread per picked receiver entry.
 __global__ void adj_mark(float *__restrict__ T, const long *__restrict__ pnode,
@@ -1226,6 +1276,16 @@ hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N,
     const long tot = N * nchunk;
     hipLaunchKernelGGL(init_fwd<PFAST_CHUNK>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        st, F, srcnode, N, nchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_src_box(hipStream_t st, float *F, const double *vp, const int *srcnode,
+                               long N, int nchunk, int R, int nrows, int ncols, int nz,
+                               double dx, double dy, double dz)
+{
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_src_box<PFAST_CHUNK>, dim3((unsigned)(nchunk * PFAST_CHUNK)),
+                       dim3(256), 0, st, F, vp, srcnode, N, R, nrows, ncols, nz, dx, dy, dz);
     return hipGetLastError();
 }
 

@@ -976,7 +976,10 @@ int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const 
 int fwd_begin3d(pfast_shots3d *s)
 {
     const long N = (long)s->m->nrows * s->m->ncols * s->m->nz;
-    HIPCHK(launch_init_fwd(s->m->ctx->stream, s->T, s->srcnode, N, s->nchunk));
+    pfast_model3d *m = s->m;
+    HIPCHK(launch_init_fwd(m->ctx->stream, s->T, s->srcnode, N, s->nchunk));
+    HIPCHK(launch_init_src_box(m->ctx->stream, s->T, m->vp, s->srcnode, N, s->nchunk, s->srcinit,
+                               m->nrows, m->ncols, m->nz, m->dx, m->dy, m->dz));
     return PFAST_OK;
 }
 
@@ -1283,6 +1286,21 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     }
     HIPCHK(hipStreamSynchronize(st));
     *out = s;
+    return PFAST_OK;
+}
+
+int pfast_shots3d_set_source_init(pfast_shots3d *s, int halfwidth)
+{
+    ARGCHK(s, "NULL shots");
+    ARGCHK(halfwidth >= 0 && halfwidth <= PFAST_SRCINIT_MAX,
+           "source box half-width outside 0.." + std::to_string(PFAST_SRCINIT_MAX));
+    hipStream_t st = s->m->ctx->stream;
+    HIPCHK(hipSetDevice(s->m->ctx->device));
+    s->srcinit = halfwidth;
+    for (ChunkRef &c : s->hchunks) c.srcinit = halfwidth;
+    HIPCHK(hipMemcpyAsync(s->chunks, s->hchunks.data(), s->hchunks.size() * sizeof(ChunkRef),
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;
 }
 

@@ -27,6 +27,7 @@ struct pfast_shots3d {
     std::vector<int> srnum;  // nsrc
     std::vector<long> ch0;   // nsrc+1 prefix
     int *srcnode = nullptr;  // [nchunk*C]
+    int srcinit = 0;         // half-width of the fixed FP64 box around each source
     float *T = nullptr;      // [nchunk][N][C] traveltimes
     float *L = nullptr;      // [nchunk][N][C][2] adjoint (lambda, T) pairs
     ChunkRef *chunks = nullptr;          // [nchunk] device table the sweep kernels read
