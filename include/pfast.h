@@ -222,6 +222,20 @@ int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable);
  * dataflow when the active chunks x tiles of an ordering are few enough that the
  * per-diagonal launches would leave the GPU idle.  Results are the same. */
 int pfast_ctx_set_dataflow(pfast_ctx *ctx, int mode);
+/* Sweep cycles of a batched 3D phase issued between two convergence checks on the
+ * host (1 = every cycle, the default; at most PFAST_CHECK_INTERVAL_MAX).  The
+ * reference tests convergence after every cycle (src/T_pfsm3d.h:181,
+ * src/T_padjoint3d.h:165), and so does the device here whatever the interval: a
+ * converged source's cycle count is the reference's, and a chunk whose sources all
+ * converged sweeps no further.  A longer interval saves host round trips at the cost
+ * of launches that return at once after the last chunk converged. */
+#define PFAST_CHECK_INTERVAL_MAX 64
+int pfast_ctx_set_check_interval(pfast_ctx *ctx, int cycles);
+/* Replay the sweep cycles between two checks from captured hipGraphs (no counterpart
+ * in the reference): captured on first use of a chunk list, schedule and interval,
+ * kept per shot set (at most 16), then launched as one graph.  Off by default and
+ * ignored while timing is on; results are the same. */
+int pfast_ctx_set_graphs(pfast_ctx *ctx, int enable);
 
 /* ---------------------------------------------------------------------------------
  * Batched 2D shot sets: every source of a 2D survey resident on one GPU (FP32

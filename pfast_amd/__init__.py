@@ -183,6 +183,16 @@ class Context:
         m = 2 if mode == "auto" else int(mode)
         check(lib().pfast_ctx_set_dataflow(self.h, m))
 
+    def set_check_interval(self, cycles):
+        """Sweep cycles of a batched 3D phase between two convergence checks on the
+        host (1..64; converged chunks retire on the device every cycle either way, so
+        results and cycle counts do not change)."""
+        check(lib().pfast_ctx_set_check_interval(self.h, int(cycles)))
+
+    def set_graphs(self, enable=True):
+        """Replay the cycles between two checks from captured hipGraphs."""
+        check(lib().pfast_ctx_set_graphs(self.h, 1 if enable else 0))
+
     def set_streams(self, n):
         """Split batched 3D cycles over n streams (1..4); results do not change."""
         check(lib().pfast_ctx_set_streams(self.h, int(n)))

@@ -38,6 +38,13 @@ struct SweepArgs {
     // change nothing: skipped.
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
+    // Phase state on the device (NULL: the host passes every cycle's values): the
+    // cycle number phase[0] (1-based) that the cycle's first kernel advanced; epoch then
+    // counts from the cycle's start, to which the kernel adds (phase[0] - 1) * ep_cycle.
+    // live[chunk] = 0 once a chunk has converged: its tiles return at once.
+    const int *phase;
+    const int *live;
+    int ep_cycle;
     // node-sweeps of the tiles actually swept (columns x depth x real sources), added up
     // on the device while profiling (NULL: not counted)
     unsigned long long *swept;
@@ -55,6 +62,10 @@ struct FlowArgs {
     int o0;              // first ordering: 0 (depth decreasing) or 4 (increasing)
     int cycle, nd;       // cycle number and tile diagonals, for the tile-skip epochs
     unsigned long long timeout;  // wall-clock ticks a dependency wait may take
+    // Phase state on the device (NULL: base and cycle as given): base and cycle then
+    // count from cycle 0, and the kernel adds (2 * phase[0]) << 16 to base and sets
+    // cycle = phase[0] - 1.
+    const int *phase;
 };
 
 // A sweep kernel variant: schedule (0: column window, the only one), tile rows x
@@ -82,6 +93,16 @@ size_t flow_words(int nchunk, int ntile);
 // captured graph.
 hipError_t launch_flow_reset(hipStream_t st, unsigned *state, size_t words);
 hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N, int nchunk);
+// Phase state on the device (SweepArgs::phase): int[2 + nchunk + nsrc] = cycle number,
+// spare, live[nchunk], cycles[nsrc] (the first cycle in which a source changed nothing,
+// 0 while it still changes).
+hipError_t launch_phase_init(hipStream_t st, int *phase, int nchunk, int nsrc);
+// Start of a cycle: advance the cycle number, clear the chunks' change bits.
+hipError_t launch_cycle_begin(hipStream_t st, int *phase, unsigned *changed, int nchunk);
+// End of a cycle: record each live chunk's converged sources and retire the chunks
+// none of whose sources changed.
+hipError_t launch_cycle_end(hipStream_t st, int *phase, const unsigned *changed, int nchunk,
+                            int nsrc);
 hipError_t launch_init_src_box(hipStream_t st, float *F, const double *vp, const int *srcnode,
                                long N, int nchunk, int R, int nrows, int ncols, int nz,
                                double dx, double dy, double dz);

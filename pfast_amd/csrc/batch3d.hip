@@ -52,6 +52,17 @@ __device__ __forceinline__ TileCtx decode_tile(const SweepArgs &a, int D, int s1
     return t;
 }
 
+// Device phase state (SweepArgs::phase): is the chunk still sweeping; the launch's
+// tile-skip epoch from the cycle number.
+__device__ __forceinline__ bool phase_live(const SweepArgs &a, int chunk)
+{
+    return !a.live || a.live[chunk] != 0;
+}
+__device__ __forceinline__ void phase_epoch(SweepArgs &a)
+{
+    if (a.phase) a.epoch += (a.phase[0] - 1) * a.ep_cycle;
+}
+
 __device__ __forceinline__ unsigned chunk_bits(unsigned long long m, int C)
 {
     // bit c set iff some lane with (lane % C) == c voted
@@ -527,9 +538,10 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
     __shared__ float pub[2 * PB];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
+    if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
+    phase_epoch(a);
     fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr, nullptr);
 }
 
@@ -709,9 +721,10 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     __shared__ float2 pub[2 * PB];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
-    if (!t.live) return;
+    if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
+    phase_epoch(a);
     adj_tile<TI, TJ, C, L, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr, nullptr);
 }
 
@@ -735,6 +748,16 @@ struct FlowTaskLds {
 
 // A value every lane read from the same LDS word, as a scalar.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Device phase state (FlowArgs::phase): this cycle's launch epoch and cycle number.
+__device__ __forceinline__ void flow_phase(FlowArgs &f)
+{
+    if (f.phase) {
+        const int cyc = f.phase[0];
+        f.base += (unsigned)(2 * cyc) << pflow::kEpochShift;
+        f.cycle = cyc - 1;
+    }
+}
 
 // Progress word of tile (ti, tj) of a chunk in ordering o.
 __device__ __forceinline__ unsigned *flow_word(const SweepArgs &a, const FlowArgs &f, int chunk,
@@ -784,7 +807,9 @@ __device__ __forceinline__ void flow_claim(const SweepArgs &a, const FlowArgs &f
                                       s.d);
             s.ndep = n;
             s.skip = 0;
-            if (a.tmod) {
+            if (!phase_live(a, s.chunk)) {
+                s.skip = 2;  // retired chunk: nothing of it runs, nothing waits on it
+            } else if (a.tmod) {
                 const int ix = (s.chunk * a.nti + k.ti) * a.ntj + k.tj;
                 const int cl = ld_agent(&a.tclean[ix * 8 + k.o]);
                 if (cl > 0 && ld_agent(&a.tmod[ix * 5]) < cl) {
@@ -800,7 +825,7 @@ __device__ __forceinline__ void flow_claim(const SweepArgs &a, const FlowArgs &f
                     s.skip = skip_tile<TI, TJ, true>(at, tc, ix) ? 1 : 0;
                 }
             }
-            if (s.skip) st_agent(flow_word(a, f, s.chunk, k.o, k.ti, k.tj), f.base + pflow::kDone);
+            if (s.skip == 1) st_agent(flow_word(a, f, s.chunk, k.o, k.ti, k.tj), f.base + pflow::kDone);
         }
     }
 }
@@ -860,6 +885,7 @@ __device__ __forceinline__ void flow_finish(const FlowArgs &f, unsigned *self)
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void fwd3d_flow(SweepArgs a, FlowArgs f)
 {
+    flow_phase(f);
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
     __shared__ int any_change;
@@ -882,6 +908,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
 template <int TI, int TJ, int C, int L, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void adj3d_flow(SweepArgs a, FlowArgs f)
 {
+    flow_phase(f);
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float2 pub[2 * PB];
     __shared__ int any_change;
@@ -914,6 +941,37 @@ __global__ void init_fwd(float *__restrict__ F, const int *__restrict__ srcnode,
 #pragma unroll
     for (int c = 0; c < C; c++)
         F[e * C + c] = srcnode[chunk * C + c] == n ? 0.0f : PFAST_INF_F;
+}
+
+// Phase state: cycle 0, every chunk live, no source converged.
+__global__ void phase_init(int *__restrict__ phase, int nchunk, int nsrc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < 2) phase[e] = 0;
+    else if (e < 2 + nchunk) phase[e] = 1;
+    else if (e < 2 + nchunk + nsrc) phase[e] = 0;
+}
+
+__global__ void cycle_begin(int *__restrict__ phase, unsigned *__restrict__ changed, int nchunk)
+{
+    for (int i = threadIdx.x; i < nchunk; i += blockDim.x) changed[i] = 0u;
+    if (threadIdx.x == 0) phase[0] += 1;
+}
+
+// A live chunk's sources that changed nothing this cycle converge now (the first such
+// cycle counts); a chunk none of whose sources changed retires.
+__global__ void cycle_end(int *__restrict__ phase, const unsigned *__restrict__ changed, int nchunk,
+                          int nsrc)
+{
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    int *live = phase + 2, *cyc = phase + 2 + nchunk;
+    if (ch >= nchunk || !live[ch]) return;
+    const unsigned bits = changed[ch];
+    for (int c = 0; c < PFAST_CHUNK; c++) {
+        const int src = ch * PFAST_CHUNK + c;
+        if (src < nsrc && !((bits >> c) & 1u) && cyc[src] == 0) cyc[src] = phase[0];
+    }
+    if (!bits) live[ch] = 0;
 }
 
 // Fixed box around each source (half-width R > 0 nodes, clipped to the grid): FP64
@@ -1276,6 +1334,27 @@ hipError_t launch_init_fwd(hipStream_t st, float *F, const int *srcnode, long N,
     const long tot = N * nchunk;
     hipLaunchKernelGGL(init_fwd<PFAST_CHUNK>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        st, F, srcnode, N, nchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_init(hipStream_t st, int *phase, int nchunk, int nsrc)
+{
+    const int n = 2 + nchunk + nsrc;
+    hipLaunchKernelGGL(phase_init, dim3((n + 255) / 256), dim3(256), 0, st, phase, nchunk, nsrc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cycle_begin(hipStream_t st, int *phase, unsigned *changed, int nchunk)
+{
+    hipLaunchKernelGGL(cycle_begin, dim3(1), dim3(256), 0, st, phase, changed, nchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_cycle_end(hipStream_t st, int *phase, const unsigned *changed, int nchunk,
+                            int nsrc)
+{
+    hipLaunchKernelGGL(cycle_end, dim3((nchunk + 255) / 256), dim3(256), 0, st, phase, changed,
+                       nchunk, nsrc);
     return hipGetLastError();
 }
 

@@ -238,6 +238,22 @@ int pfast_ctx_set_dataflow(pfast_ctx *ctx, int mode)
     return PFAST_OK;
 }
 
+int pfast_ctx_set_check_interval(pfast_ctx *ctx, int cycles)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ARGCHK(cycles >= 1 && cycles <= PFAST_CHECK_INTERVAL_MAX,
+           "check interval outside 1.." + std::to_string(PFAST_CHECK_INTERVAL_MAX));
+    ctx->check_every = cycles;
+    return PFAST_OK;
+}
+
+int pfast_ctx_set_graphs(pfast_ctx *ctx, int enable)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ctx->graphs = enable ? 1 : 0;
+    return PFAST_OK;
+}
+
 int pfast_ctx_timing_enable(pfast_ctx *ctx, int enable)
 {
     ARGCHK(ctx, "ctx is NULL");
@@ -703,15 +719,110 @@ int pfast_radjoint_fsm2d(double *UR, const double *Td, const double *Tu, const d
 // ======================================================= batched 3D
 namespace {
 
-// Sweep cycles of the batched FP32 solver until every source is stationary.
-int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &cyc)
+// One batch of sweep cycles of a phase on the model's stream: per cycle, the cycle's
+// start (cycle number, change bits), the sweeps of every chunk in the device list(s)
+// (dataflow launches, or per tile anti-diagonal split over the context's streams as
+// `off` says), and the cycle's end (converged sources, retired chunks), all reading the
+// cycle number and the live chunks from the device phase state.
+int issue_batch3d(pfast_shots3d *s, bool adjoint, bool flow, int ncycles, int nlist,
+                  const std::vector<int> &off, const pfast::TileSkip *sp)
 {
     pfast_model3d *m = s->m;
     pfast_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
     const pfast::Grid3 g = pfast::grid_of(m);
-    if (!s->hchanged && hipHostMalloc((void **)&s->hchanged, s->nchunk * sizeof(unsigned)) != hipSuccess) {
-        s->hchanged = nullptr;
+    const int K = (int)off.size() - 1;
+    for (int c = 0; c < ncycles; c++) {
+        HIPCHK(pfast::launch_cycle_begin(st, s->phase, s->changed, s->nchunk));
+        if (flow) {
+            int rc = pfast::issue_cycle3d_flow(ctx, g, s->chunks, s->active, nlist, adjoint,
+                                               s->changed, s->flow, s->nchunk, 0, st, sp, s->phase);
+            if (rc) return rc;
+        } else {
+            if (K > 1) {
+                HIPCHK(hipEventRecord(ctx->fork_join[0], st));
+                for (int k = 1; k < K; k++) HIPCHK(hipStreamWaitEvent(ctx->aux[k - 1], ctx->fork_join[0], 0));
+            }
+            for (int k = 0; k < K; k++) {
+                hipStream_t sk = k ? ctx->aux[k - 1] : st;
+                const int *lk = s->active + off[k];
+                const int nk = off[k + 1] - off[k];
+                int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp, s->phase)
+                                 : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp, s->phase);
+                if (rc) return rc;
+            }
+            for (int k = 1; k < K; k++) {
+                HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->aux[k - 1]));
+                HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
+            }
+        }
+        HIPCHK(pfast::launch_cycle_end(st, s->phase, s->changed, s->nchunk, s->nsrc));
+    }
+    return PFAST_OK;
+}
+
+constexpr size_t kMaxGraphs = 16;  // captured batches kept per shot set
+
+void drop_graphs(pfast_shots3d *s)
+{
+    for (auto &gr : s->graphs) (void)hipGraphExecDestroy(gr.exec);
+    s->graphs.clear();
+}
+
+// The batch as a hipGraph: captured on first use of its key (everything its launch
+// arguments depend on), replayed afterwards.
+int launch_batch3d_graph(pfast_shots3d *s, bool adjoint, bool flow, int ncycles, int nlist,
+                         const std::vector<int> &off, const std::vector<int> &lists,
+                         const pfast::TileSkip *sp)
+{
+    pfast_ctx *ctx = s->m->ctx;
+    hipStream_t st = ctx->stream;
+    std::vector<long> key = {adjoint, flow, ncycles, nlist, ctx->sweep, ctx->tile_skip,
+                             sp != nullptr, (long)(size_t)s->flow, (long)(size_t)s->tmod};
+    key.insert(key.end(), off.begin(), off.end());
+    key.insert(key.end(), lists.begin(), lists.end());
+    for (auto &gr : s->graphs)
+        if (gr.key == key) {
+            HIPCHK(hipGraphLaunch(gr.exec, st));
+            return PFAST_OK;
+        }
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = issue_batch3d(s, adjoint, flow, ncycles, nlist, off, sp);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHK(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(ei);
+    if (s->graphs.size() >= kMaxGraphs) {
+        (void)hipGraphExecDestroy(s->graphs.front().exec);
+        s->graphs.erase(s->graphs.begin());
+    }
+    s->graphs.push_back({std::move(key), exec});
+    HIPCHK(hipGraphLaunch(exec, st));
+    return PFAST_OK;
+}
+
+// Sweep cycles of the batched FP32 solver until every source is stationary.  The host
+// checks convergence every ctx->check_every cycles; in between, a chunk that has
+// converged retires on the device at the end of its cycle (its later launches return
+// at once), so the fields, cycle counts and results do not depend on the interval.
+int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &cyc)
+{
+    pfast_model3d *m = s->m;
+    pfast_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    if (!s->phase) {
+        int rc = pfast::dalloc(ctx, &s->phase, (size_t)2 + s->nchunk + s->nsrc);
+        if (rc) return rc;
+    }
+    if (!s->hphase && hipHostMalloc((void **)&s->hphase, ((size_t)2 + s->nchunk + s->nsrc) * sizeof(int)) != hipSuccess) {
+        s->hphase = nullptr;
         pfast::set_error("hipHostMalloc failed");
         return PFAST_ERR_NOMEM;
     }
@@ -725,6 +836,7 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
     if (ctx->tile_skip && sv.kind == 0) {
         const int ntile = ((m->nrows + sv.ti - 1) / sv.ti) * ((m->ncols + sv.tj - 1) / sv.tj);
         if (ntile != s->ntile) {
+            drop_graphs(s);
             pfast::dfree(ctx, s->tmod);
             pfast::dfree(ctx, s->tclean);
             s->tmod = s->tclean = nullptr;
@@ -740,72 +852,52 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
         skip.tmod = s->tmod;
         skip.tclean = s->tclean;
     }
+    const pfast::TileSkip *sp = skip.tmod ? &skip : nullptr;
+    HIPCHK(pfast::launch_phase_init(st, s->phase, s->nchunk, s->nsrc));
     const int ftile = ((m->nrows + sv.ti - 1) / sv.ti) * ((m->ncols + sv.tj - 1) / sv.tj);
     const size_t fwords = pfast::flow_words(s->nchunk, ftile);
     bool flow_ready = false;  // dataflow state allocated and reset for this phase
-    int epoch = 0;
+    std::vector<int> lists, off;
     while (!act.empty() && cycle < MAXSwp) {
-        cycle++;
-        skip.cycle = cycle - 1;
-        const pfast::TileSkip *sp = skip.tmod ? &skip : nullptr;
-        HIPCHK(hipMemsetAsync(s->changed, 0, s->nchunk * sizeof(unsigned), st));
+        const int ncyc = std::max(1, std::min(ctx->check_every, MAXSwp - cycle));
         long real = 0;
         for (int ch : act) real += std::min(PFAST_CHUNK, s->nsrc - ch * PFAST_CHUNK);
-        // both schedules update every node in the same order: the choice is per cycle
-        const bool flow = pfast::use_dataflow(ctx, (int)act.size(), ftile, fwords);
+        // both schedules update every node in the same order: the choice is per batch;
+        // dataflow launch epochs come from the cycle number (16 bits, two per cycle)
+        const bool flow = pfast::use_dataflow(ctx, (int)act.size(), ftile, fwords) &&
+                          cycle + ncyc <= 0x7FFE;
         const int kind = flow ? (adjoint ? pfast::kAdjFlow : pfast::kFwdFlow)
                               : (adjoint ? pfast::kAdjDiag : pfast::kFwdDiag);
-        ctx->units[kind] += 8.0 * (double)((long)m->nrows * m->ncols * m->nz) * real;
-        if (flow) {
-            if (!flow_ready) {
-                if (fwords != s->nflow) {
-                    pfast::dfree(ctx, s->flow);
-                    s->flow = nullptr;
-                    s->nflow = 0;
-                    int rc = pfast::dalloc(ctx, &s->flow, fwords);
-                    if (rc) return rc;
-                    s->nflow = fwords;
-                }
-                HIPCHK(pfast::launch_flow_reset(st, s->flow, fwords));
-                flow_ready = true;
-                epoch = 0;
-            }
-            if (++epoch > 0x7FFF) {  // progress words carry 16 bits of launch epoch (2 per cycle)
-                HIPCHK(pfast::launch_flow_reset(st, s->flow, fwords));
-                epoch = 1;
-            }
-            HIPCHK(hipMemcpyAsync(s->active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            int rc = pfast::issue_cycle3d_flow(ctx, g, s->chunks, s->active, (int)act.size(),
-                                               adjoint, s->changed, s->flow, s->nchunk, epoch, st, sp);
-            if (rc) return rc;
-        } else {
-            // the active chunks, round-robin over the context's streams (one list each)
-            const int K = std::max(1, std::min(ctx->nstreams, (int)act.size()));
-            std::vector<int> lists;
-            std::vector<int> off(K + 1, 0);
-            for (int k = 0; k < K; k++) {
-                for (size_t i = k; i < act.size(); i += K) lists.push_back(act[i]);
-                off[k + 1] = (int)lists.size();
-            }
-            HIPCHK(hipMemcpyAsync(s->active, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            if (K > 1) {
-                HIPCHK(hipEventRecord(ctx->fork_join[0], st));
-                for (int k = 1; k < K; k++) HIPCHK(hipStreamWaitEvent(ctx->aux[k - 1], ctx->fork_join[0], 0));
-            }
-            for (int k = 0; k < K; k++) {
-                hipStream_t sk = k ? ctx->aux[k - 1] : st;
-                const int *lk = s->active + off[k];
-                const int nk = off[k + 1] - off[k];
-                int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp)
-                                 : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp);
+        ctx->units[kind] += 8.0 * ncyc * (double)((long)m->nrows * m->ncols * m->nz) * real;
+        if (flow && !flow_ready) {
+            if (fwords != s->nflow) {
+                drop_graphs(s);
+                pfast::dfree(ctx, s->flow);
+                s->flow = nullptr;
+                s->nflow = 0;
+                int rc = pfast::dalloc(ctx, &s->flow, fwords);
                 if (rc) return rc;
+                s->nflow = fwords;
             }
-            for (int k = 1; k < K; k++) {
-                HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->aux[k - 1]));
-                HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
-            }
+            HIPCHK(pfast::launch_flow_reset(st, s->flow, fwords));
+            flow_ready = true;
         }
-        HIPCHK(hipMemcpyAsync(s->hchanged, s->changed, s->nchunk * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        // the live chunks: one list for the dataflow launches, else round-robin over the
+        // context's streams (one list each)
+        const int K = flow ? 1 : std::max(1, std::min(ctx->nstreams, (int)act.size()));
+        lists.clear();
+        off.assign(K + 1, 0);
+        for (int k = 0; k < K; k++) {
+            for (size_t i = k; i < act.size(); i += K) lists.push_back(act[i]);
+            off[k + 1] = (int)lists.size();
+        }
+        HIPCHK(hipMemcpyAsync(s->active, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        int rc = ctx->graphs && !ctx->timing
+                     ? launch_batch3d_graph(s, adjoint, flow, ncyc, (int)act.size(), off, lists, sp)
+                     : issue_batch3d(s, adjoint, flow, ncyc, (int)act.size(), off, sp);
+        if (rc) return rc;
+        cycle += ncyc;
+        HIPCHK(hipMemcpyAsync(s->hphase, s->phase, (2 + (size_t)s->nchunk) * sizeof(int), hipMemcpyDeviceToHost, st));
         int aborted = 0;
         if (flow)
             HIPCHK(hipMemcpyAsync(&aborted, s->flow + fwords - 2, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -816,18 +908,16 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
             return PFAST_ERR_HIP;
         }
         std::vector<int> next;
-        for (int ch : act) {
-            const unsigned bits = s->hchanged[ch];
-            for (int c = 0; c < PFAST_CHUNK; c++) {
-                const int src = ch * PFAST_CHUNK + c;
-                if (src < s->nsrc && !((bits >> c) & 1u) && cyc[src] == 0) cyc[src] = cycle;
-            }
-            if (bits) next.push_back(ch);
-        }
+        for (int ch : act)
+            if (s->hphase[2 + ch]) next.push_back(ch);
         act.swap(next);
     }
-    for (int i = 0; i < s->nsrc; i++)
-        if (cyc[i] == 0) cyc[i] = cycle;
+    HIPCHK(hipMemcpyAsync(s->hphase, s->phase, ((size_t)2 + s->nchunk + s->nsrc) * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < s->nsrc; i++) {
+        const int c = s->hphase[2 + s->nchunk + i];
+        cyc[i] = c ? c : cycle;
+    }
     return PFAST_OK;
 }
 
@@ -890,7 +980,7 @@ bool use_dataflow(const pfast_ctx *ctx, int nact, int ntile, size_t words)
 
 int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *act,
                        int nact, bool adjoint, unsigned *changed, unsigned *flow, int nids,
-                       int epoch, hipStream_t st, const TileSkip *skip)
+                       int epoch, hipStream_t st, const TileSkip *skip, const int *dphase)
 {
     if (!st) st = ctx->stream;
     if (nact <= 0) return PFAST_OK;
@@ -907,12 +997,17 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
     f.ticket = flow + nprog;
     f.abort = reinterpret_cast<int *>(flow + nprog + 2);
     f.ntasks = 4u * (unsigned)ntile * (unsigned)nact;
-    f.cycle = skip ? skip->cycle : 0;
+    f.cycle = skip && !dphase ? skip->cycle : 0;
     f.nd = a.nti + a.ntj - 1;
     f.timeout = flow_timeout_ticks(ctx->device);
+    if (dphase) {
+        f.phase = dphase;
+        a.live = dphase + 2;
+    }
     for (int half = 0; half < 2; half++) {  // orderings 0-3, then 4-7
         f.o0 = 4 * half;
-        f.base = (unsigned)(2 * epoch - 1 + half) << pflow::kEpochShift;
+        // with dphase the kernel adds (2 * cycle) << 16 (unsigned arithmetic wraps)
+        f.base = (unsigned)((dphase ? 0 : 2 * epoch) - 1 + half) << pflow::kEpochShift;
         time_begin(ctx, adjoint ? kAdjFlow : kFwdFlow, st);
         HIPCHK(launch_sweep3d_flow(st, a, f, ctx->sweep, adjoint ? 1 : 0, g.uniform));
         time_end(ctx, st);
@@ -922,7 +1017,7 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
 
 int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
                       int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
-                      hipStream_t st, const TileSkip *skip)
+                      hipStream_t st, const TileSkip *skip, const int *dphase)
 {
     if (!st) st = ctx->stream;
     const SweepVariant &sv = sweep_variant(ctx->sweep);
@@ -930,6 +1025,13 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     const int nti = a.nti, ntj = a.ntj;
     const bool skipping = a.tmod != nullptr;
     a.o = o;
+    if (dphase) {
+        a.phase = dphase;
+        a.live = dphase + 2;
+        a.ep_cycle = 8 * (nti + ntj - 1);
+    }
+    // with dphase the kernels add (cycle - 1) * ep_cycle to the epoch
+    const int ecyc = dphase || !skipping ? 0 : skip->cycle;
     SweepArgs af = a, aa = a;
     if (ctx->timing && ctx->dswept) {
         af.swept = ctx->dswept + kFwdDiag;
@@ -945,7 +1047,7 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     const int nd = nti + ntj - 1;
     for (int D = D0; D < D1; D++) {
         const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
-        if (skipping) af.epoch = aa.epoch = 1 + ((skip->cycle * 8 + o) * nd + D);
+        if (skipping) af.epoch = aa.epoch = 1 + ((ecyc * 8 + o) * nd + D);
         if (nfwd) {
             time_begin(ctx, kFwdDiag, st);
             HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
@@ -962,12 +1064,12 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
 
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
                   int nfwd, const int *adj, int nadj, unsigned *changed, int o0, int o1,
-                  hipStream_t st, const TileSkip *skip)
+                  hipStream_t st, const TileSkip *skip, const int *dphase)
 {
     const int nd = diagonals3d(ctx, g);
     for (int o = o0; o < o1; o++) {
         int rc = issue_diagonals3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, nd, st,
-                                   skip);
+                                   skip, dphase);
         if (rc) return rc;
     }
     return PFAST_OK;
@@ -1145,9 +1247,10 @@ void shots3d_free(pfast_shots3d *s)
     void *ptrs[] = {s->srcnode, s->T, s->L, s->rnode, s->rsrc, s->rec3, s->nxyz, s->Tobs,
                     s->pnode, s->psrc, s->pobs, s->poff, s->bnode, s->bsrc, s->bobs, s->bnrm,
                     s->EE, s->grad, s->picks, s->fieldbuf, s->changed, s->active, s->chunks,
-                    s->tmod, s->tclean, s->flow};
+                    s->tmod, s->tclean, s->flow, s->phase};
+    drop_graphs(s);
     for (void *p : ptrs) pfast::dfree(s->m->ctx, p);
-    if (s->hchanged) (void)hipHostFree(s->hchanged);
+    if (s->hphase) (void)hipHostFree(s->hphase);
     delete s;
 }
 

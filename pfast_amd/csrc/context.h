@@ -46,6 +46,11 @@ struct pfast_ctx {
     int nstreams = 1;
     int tile_skip = 1;  // exact tile skipping in the column-window sweeps
     int dataflow = 2;   // batched 3D cycles as dataflow launches: 0 never, 1 always, 2 auto
+    // batched 3D phases: sweep cycles issued between two convergence checks on the host
+    // (a converged chunk retires on the device at the end of its cycle either way), and
+    // whether those cycles are replayed from a captured hipGraph
+    int check_every = 1;
+    int graphs = 0;
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> fork_join;  // [0] fork on stream, [1..] joins on aux
 };

@@ -55,7 +55,15 @@ struct pfast_shots3d {
     double *EE = nullptr, *grad = nullptr, *picks = nullptr, *fieldbuf = nullptr;
     unsigned *changed = nullptr;
     int *active = nullptr;
-    unsigned *hchanged = nullptr;
+    // phase state on the device (launch_phase_init) and its host mirror (pinned)
+    int *phase = nullptr;
+    int *hphase = nullptr;
+    // captured sweep cycles, keyed by everything their launch arguments depend on
+    struct Graph {
+        std::vector<long> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
     std::vector<int> fcyc, acyc;
     bool have_adj_data = false;
 };
@@ -85,13 +93,17 @@ struct TileSkip {
 };
 
 // st: the stream to issue on (nullptr: ctx->stream); skip: NULL for no skipping.
+// dphase: the device phase state (SweepArgs::phase) the launches read their cycle
+// number and live chunks from, or NULL (the cycle number in skip, every chunk live).
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
                   int nfwd, const int *adj, int nadj, unsigned *changed, int o0 = 0, int o1 = 8,
-                  hipStream_t st = nullptr, const TileSkip *skip = nullptr);
+                  hipStream_t st = nullptr, const TileSkip *skip = nullptr,
+                  const int *dphase = nullptr);
 // Tile anti-diagonals [D0, D1) of ordering o (same arguments otherwise).
 int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
                       int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
-                      hipStream_t st = nullptr, const TileSkip *skip = nullptr);
+                      hipStream_t st = nullptr, const TileSkip *skip = nullptr,
+                      const int *dphase = nullptr);
 // Active chunks x tiles of one ordering up to which the automatic schedule takes the
 // dataflow launches: they keep the GPU busy where a launch per tile anti-diagonal
 // has too few tiles (measured on 201x201x101: 2.0x faster with 64 sources, 1.4x with
@@ -105,10 +117,12 @@ bool use_dataflow(const pfast_ctx *ctx, int nact, int ntile, size_t words);
 // stream st over the chunk ids act[0..nact) (device array) of the chunk table,
 // forward or adjoint.  flow: the phase's dataflow state, flow_words(id bound, tiles)
 // words, reset at the phase start; epoch: this cycle's number in the phase (1, 2, ...,
-// at most 0x7FFF); skip as above.
+// at most 0x7FFF); skip as above.  With dphase the launches take the cycle number
+// from the device phase state instead (epoch unused; at most 0x7FFF cycles per reset).
 int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *act,
                        int nact, bool adjoint, unsigned *changed, unsigned *flow, int nids,
-                       int epoch, hipStream_t st, const TileSkip *skip);
+                       int epoch, hipStream_t st, const TileSkip *skip,
+                       const int *dphase = nullptr);
 // Tile anti-diagonals of one ordering under ctx's sweep variant.
 int diagonals3d(const pfast_ctx *ctx, const Grid3 &g);
 
