@@ -257,9 +257,11 @@ def driver3d(cfg, iters=3):
         F.write_inp3d(os.path.join(d, "m.inp"), "m.vp", "m.src", cfg["src_tab"].shape, "m.rec",
                       cfg["rec_tab"].shape, nr, nc, nz, h, h, h, vpfile="final.vp")
         t0 = time.perf_counter()
+        # --no-autotune: the wall time of the iterations themselves (the driver's own
+        # tuning, on by default, times the same candidates the bench tuned above)
         r = subprocess.run([os.path.join(ROOT, "pfast_amd", "bin", "pfast_t3d"), "m.inp", "5",
-                            "--max-iter", str(iters)], cwd=d, capture_output=True, text=True,
-                           timeout=600)
+                            "--max-iter", str(iters), "--no-autotune"], cwd=d,
+                           capture_output=True, text=True, timeout=600)
         wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(r.stdout[-1000:] + r.stderr[-1000:])
@@ -289,9 +291,20 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (3 times), the "
                          "known-size kernel that scales the HBM byte counters of a profile")
-    ap.add_argument("--autotune", action="store_true",
-                    help="choose the sweep variant by timing every variant on this shot set "
-                         "first (results are the same under each)")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=True,
+                    help="(default) before the warm-up, choose the sweep variant (tile shape "
+                         "and batch depths), then the schedule and the convergence-check "
+                         "interval, by timing them on this shot set (results are the same "
+                         "under each)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep --sweep, --schedule and --check-interval as given")
+    ap.add_argument("--check-interval", type=int, default=1,
+                    help="sweep cycles between two convergence checks on the host")
+    ap.add_argument("--graphs", choices=["on", "off"], default="off",
+                    help="replay the cycles between checks from captured hipGraphs (the "
+                         "bench line also carries the step time with the other setting)")
+    ap.add_argument("--ab-steps", type=int, default=2,
+                    help="steps timed with hipGraphs toggled, for the graph A/B (0: skip)")
     ap.add_argument("--schedule", choices=["auto", "dataflow", "diagonal"], default="auto",
                     help="batched 3D sweep schedule: dataflow launches (two per cycle), one "
                          "launch per tile anti-diagonal, or per cycle the one that suits its "
@@ -349,17 +362,29 @@ def main():
     ctx.set_tile_skip(not args.no_tile_skip)
     ctx.set_dataflow({"auto": 2, "dataflow": 1, "diagonal": 0}[args.schedule])
     ctx.set_sweep(args.sweep)
+    ctx.set_check_interval(args.check_interval)
+    ctx.set_graphs(args.graphs == "on")
     sweep_name = pfast_amd.sweep_variants()[args.sweep]
     log(f"sweep kernels: {sweep_name}")
     model = pfast_amd.Model3D(ctx, vp, h, h, h)
     shots = pfast_amd.Shots3D(model, cfg["src"], cfg["srnum"], cfg["rec"], cfg["nxyz"], cfg["T_obs"])
     ctx.sync()
     log("inputs resident in HBM")
+    tune = None
     if args.autotune:
+        t = time.perf_counter()
         args.sweep, ms = shots.autotune()
         sweep_name = pfast_amd.sweep_variants()[args.sweep]
-        log("autotune: " + ", ".join(f"{v} {t:.0f} ms" for v, t in ms.items()) +
+        log("autotune variants: " + ", ".join(f"{v} {t:.0f} ms" for v, t in ms.items()) +
             f" -> {sweep_name}")
+        (sched, args.check_interval), ms2 = shots.autotune_schedule()
+        args.schedule = {0: "diagonal", 1: "dataflow", 2: "auto"}[sched]
+        log("autotune (schedule, check interval): " +
+            ", ".join(f"{k} {v:.0f} ms" for k, v in ms2.items()) +
+            f" -> {args.schedule}, every {args.check_interval} cycle(s)")
+        tune = {"variant_ms": {pfast_amd.sweep_variants()[v]: round(t_, 1) for v, t_ in ms.items()},
+                "schedule_ms": {f"{k[0]},{k[1]}": round(v, 1) for k, v in ms2.items()},
+                "seconds": round(time.perf_counter() - t, 1)}
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -380,6 +405,25 @@ def main():
     grad, misfit = shots.gradient()
     log(f"timed: {ms_step:.1f} ms/step, {value:.1f} source-solves/s; FP32 cycles fwd "
         f"{cyc[:, 0].mean():.2f} adj {cyc[:, 1].mean():.2f}; misfit {misfit:.6g}")
+
+    # A/B: the same steps with hipGraph replay of the cycles toggled (one untimed step
+    # first, which captures the graphs when they are turned on)
+    graphs_ab = None
+    if args.ab_steps > 0:
+        on = args.graphs == "on"
+        ctx.set_graphs(not on)
+        shots.run(adjoint=True, MAXSwp=args.maxswp)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(args.ab_steps):
+            shots.run(adjoint=True, MAXSwp=args.maxswp)
+        ctx.sync()
+        other = 1000.0 * (time.perf_counter() - t) / args.ab_steps
+        ctx.set_graphs(on)
+        graphs_ab = {"on_ms": round(ms_step if on else other, 2),
+                     "off_ms": round(other if on else ms_step, 2),
+                     "timed_setting": args.graphs, "ab_steps": args.ab_steps}
+        log(f"hipGraph A/B: on {graphs_ab['on_ms']:.1f} ms/step, off {graphs_ab['off_ms']:.1f} ms/step")
 
     # one more step with per-launch HIP events, launches one at a time on the library's
     # stream, so that a launch's duration is its own (the roofline of the kernel).  Per
@@ -452,10 +496,13 @@ def main():
                                f"{args.nsrc} sources x {args.nrec} receivers, forward+adjoint+gradient",
                    "nsrc": args.nsrc, "nrec": args.nrec, "grid": [101, 201, 201], "h": h,
                    "parallelism": "all sources batched on 1 GPU", "sweep": sweep_name,
-                   "schedule": args.schedule,
+                   "schedule": args.schedule, "check_interval": args.check_interval,
+                   "graphs": args.graphs, "autotuned": bool(args.autotune),
                    "streams": args.streams,
                    "tile_skip": not args.no_tile_skip},
         "roofline": roofline,
+        "graphs": graphs_ab,
+        "autotune": tune,
         "kernels": {n: {kk: v for kk, v in k.items() if not kk.startswith("bytes")}
                     for n, k in kern.items()},
         "fp32_cycles_mean": [round(float(cyc[:, 0].mean()), 3), round(float(cyc[:, 1].mean()), 3)],

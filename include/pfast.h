@@ -151,12 +151,22 @@ int pfast_shots3d_set_source_init(pfast_shots3d *s, int halfwidth);
 /* Sweep-variant tuning (no counterpart in the reference, whose sweeps have one fixed
  * loop order): times two forward cycles of this shot set under each listed variant
  * (all when variants is NULL), and two adjoint cycles after them when the set has
- * picks, sets the fastest on the shot set's context and returns it in *chosen, with
+ * picks, sets the fastest on the shot set's context (the variant in use stays unless
+ * another is at least 2% faster) and returns it in *chosen, with
  * each candidate's time in ms[i] when ms is not NULL.  Every variant gives
  * bit-identical results; the shot set's fields are left mid-solve, so run it again
  * afterwards. */
 int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants, int *chosen,
                            double *ms);
+/* Schedule tuning (no counterpart in the reference): under the context's sweep variant,
+ * times a full solve of this shot set (forward, and adjoint when it has picks) under
+ * five candidates (schedule, check interval) = (2,1), (0,1), (1,1), (2,2), (2,4) -- see
+ * pfast_ctx_set_dataflow and pfast_ctx_set_check_interval -- after a one-cycle warm-up
+ * each, sets the fastest on the context (the setting in use stays unless another is
+ * at least 2% faster) and returns it; ms[5] receives the times when
+ * not NULL.  Results are the same under every candidate; run the shot set again
+ * afterwards (its fields hold the last candidate's solve). */
+int pfast_shots3d_autotune_schedule(pfast_shots3d *s, int *schedule, int *interval, double *ms);
 
 /* Fetch results.  grad: N doubles (sum over sources of -lambda/v^3, the
  * reference's global gradient before regularisation).  misfit: sum of EE.

@@ -303,6 +303,17 @@ class Shots3D:
         keys = range(n) if cand is None else cand.tolist()
         return chosen.value, dict(zip(keys, ms.tolist()))
 
+    SCHEDULE_CANDIDATES = ((2, 1), (0, 1), (1, 1), (2, 2), (2, 4))
+
+    def autotune_schedule(self):
+        """Time a full solve under each (schedule, check interval) candidate, set the
+        fastest on the context and return ((schedule, interval), {candidate: ms}).
+        Results are the same under every candidate; run the shot set afterwards."""
+        ms = np.zeros(len(self.SCHEDULE_CANDIDATES))
+        sc, iv = C.c_int(0), C.c_int(0)
+        check(lib().pfast_shots3d_autotune_schedule(self.h, C.byref(sc), C.byref(iv), ptr(ms)))
+        return (sc.value, iv.value), dict(zip(self.SCHEDULE_CANDIDATES, ms.tolist()))
+
     def gradient(self):
         N = int(np.prod(self.model.shape))
         g = np.empty(N)

@@ -73,6 +73,7 @@ _SIGS = {
     "pfast_shots3d_run": [vpp, i, i],
     "pfast_shots3d_set_source_init": [vpp, i],
     "pfast_shots3d_autotune": [vpp, vpp, i, pint, vpp],
+    "pfast_shots3d_autotune_schedule": [vpp, pint, pint, vpp],
     "pfast_shots3d_gradient": [vpp, vpp, pdbl],
     "pfast_shots3d_picks": [vpp, dp],
     "pfast_shots3d_field": [vpp, i, i, dp],

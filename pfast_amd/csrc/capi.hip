@@ -1426,6 +1426,9 @@ int pfast_shots3d_run(pfast_shots3d *s, int mode, int MAXSwp)
     return PFAST_OK;
 }
 
+// A tuner replaces the setting in use only by a candidate at least 2% faster.
+constexpr double kTuneMargin = 1.02;
+
 int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants, int *chosen,
                            double *ms)
 {
@@ -1447,7 +1450,7 @@ int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants,
     // the adjoint batch depth apart)
     const int keep = ctx->sweep;
     int best = keep;
-    double tbest = 0.0;
+    double tbest = 0.0, tkeep = 0.0;
     for (size_t i = 0; i < cand.size(); i++) {
         ctx->sweep = cand[i];
         int rc = pfast_shots3d_run(s, PFAST_FWD_ONLY, 1);
@@ -1471,13 +1474,58 @@ int pfast_shots3d_autotune(pfast_shots3d *s, const int *variants, int nvariants,
         const double dt =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (ms) ms[i] = dt;
+        if (cand[i] == keep) tkeep = dt;
         if (i == 0 || dt < tbest) {
             tbest = dt;
             best = cand[i];
         }
     }
+    // the variant in use stays unless another is clearly faster (timing noise is ~1%)
+    if (tkeep > 0.0 && tkeep <= kTuneMargin * tbest) best = keep;
     ctx->sweep = best;
     *chosen = best;
+    return PFAST_OK;
+}
+
+int pfast_shots3d_autotune_schedule(pfast_shots3d *s, int *schedule, int *interval, double *ms)
+{
+    ARGCHK(s && schedule && interval, "NULL argument");
+    pfast_ctx *ctx = s->m->ctx;
+    // the schedule decides which launches a cycle takes, the interval how many cycles
+    // pass between host checks: both act over a whole phase, so each candidate is timed
+    // on a full forward solve (and a full adjoint one with picks); results never change
+    const int keep_df = ctx->dataflow, keep_iv = ctx->check_every;
+    const int cand[][2] = {{2, 1}, {0, 1}, {1, 1}, {2, 2}, {2, 4}};
+    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    int best = 0, cur = -1;
+    double tbest = 0.0;
+    std::vector<double> t(ncand);
+    for (int i = 0; i < ncand; i++) {
+        if (cand[i][0] == keep_df && cand[i][1] == keep_iv) cur = i;
+        ctx->dataflow = cand[i][0];
+        ctx->check_every = cand[i][1];
+        int rc = pfast_shots3d_run(s, PFAST_FWD_ONLY, 1);  // warm-up (first use of the schedule)
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!rc) rc = pfast_shots3d_run(s, s->have_adj_data ? PFAST_FWD_ADJ : PFAST_FWD_ONLY, 100);
+        if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PFAST_ERR_HIP;
+        if (rc) {
+            ctx->dataflow = keep_df;
+            ctx->check_every = keep_iv;
+            return rc;
+        }
+        const double dt =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms) ms[i] = dt;
+        t[i] = dt;
+        if (i == 0 || dt < tbest) {
+            tbest = dt;
+            best = i;
+        }
+    }
+    // the setting in use stays unless another is clearly faster
+    if (cur >= 0 && t[cur] <= kTuneMargin * tbest) best = cur;
+    ctx->dataflow = *schedule = cand[best][0];
+    ctx->check_every = *interval = cand[best][1];
     return PFAST_OK;
 }
 

@@ -4,10 +4,13 @@
 // batched GPU call over all sources (pfast_shots3d_*: FP32 sweeps, FP64 gradient
 // and misfit) and the gradient smoothing on the GPU (pfast_regu3d).
 //
-// usage: pfast_t3d input.inp [LM=5] [--forward] [--max-iter K]
+// usage: pfast_t3d input.inp [LM=5] [--forward] [--max-iter K] [--no-autotune]
 //   --forward     forward modelling only (the reference's -DNoInverse build):
 //                 writes <prefix>_Tr3D.bin
 //   --max-iter K  stop after K iterations (default 100, the reference's cap)
+//   --no-autotune keep the default sweep variant and schedule (an inversion otherwise
+//                 first times them on its shot set: pfast_shots3d_autotune and
+//                 pfast_shots3d_autotune_schedule; results are the same)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -22,15 +25,16 @@ using drv::check;
 int main(int argc, char **argv)
 {
     std::vector<const char *> pos;
-    bool forward = false;
+    bool forward = false, autotune = true;
     int MAXIter = 100;
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "--forward")) forward = true;
+        else if (!strcmp(argv[a], "--no-autotune")) autotune = false;
         else if (!strcmp(argv[a], "--max-iter") && a + 1 < argc) MAXIter = atoi(argv[++a]);
         else pos.push_back(argv[a]);
     }
     if (pos.empty() || pos.size() > 2) {
-        printf(" usage: pfast_t3d input_file [L-BFGS memory, default 5] [--forward] [--max-iter K]\n");
+        printf(" usage: pfast_t3d input_file [L-BFGS memory, default 5] [--forward] [--max-iter K] [--no-autotune]\n");
         return 1;
     }
     const int LM = pos.size() == 2 ? atoi(pos[1]) : 5;
@@ -146,6 +150,15 @@ int main(int argc, char **argv)
         pfast_model3d_destroy(model);
         pfast_ctx_destroy(ctx);
         return 0;
+    }
+
+    if (autotune) {  // every candidate gives the same results: the choice is by time
+        int variant = 0, schedule = 0, interval = 0;
+        check(pfast_shots3d_autotune(shots, nullptr, 0, &variant, nullptr), "pfast_shots3d_autotune");
+        check(pfast_shots3d_autotune_schedule(shots, &schedule, &interval, nullptr),
+              "pfast_shots3d_autotune_schedule");
+        printf(" * autotune: sweep variant %d (%s), schedule %d, convergence check every %d "
+               "cycle(s)\n", variant, pfast_sweep_name(variant), schedule, interval);
     }
 
     drv::Optimiser opt;

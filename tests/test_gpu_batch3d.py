@@ -244,7 +244,8 @@ def test_chunk_field_bound():
 
 def test_autotune_picks_fastest_candidate_results_unchanged():
     """pfast_shots3d_autotune times the listed variants, leaves the fastest set on the
-    context, and the shot set run afterwards gives the same bits as under the default."""
+    context (the variant in use, 0, unless another is at least 2% faster), and the shot
+    set run afterwards gives the same bits as under the default."""
     import pfast_amd
     c = cases.case3d(47, shape=(17, 29, 33), spacing=(25.0, 25.0, 25.0), nsrc=11, nrec_per=9)
     ctx = pfast_amd.Context(0)
@@ -256,7 +257,8 @@ def test_autotune_picks_fastest_candidate_results_unchanged():
     chosen, ms = sh.autotune([0, 1, 4])
     print(f"autotune: {ms} -> {chosen}", flush=True)
     assert sorted(ms) == [0, 1, 4] and all(t > 0 for t in ms.values())
-    assert ms[chosen] == min(ms.values())
+    fastest = min(ms, key=ms.get)
+    assert chosen == (0 if ms[0] <= 1.02 * ms[fastest] else fastest)
     sh.run(adjoint=True, MAXSwp=100)
     out = (np.stack([sh.field(s) for s in range(11)]),
            np.stack([sh.field(s, adjoint=True) for s in range(11)])) + sh.gradient()

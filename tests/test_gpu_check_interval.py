@@ -101,3 +101,26 @@ def test_check_interval_out_of_range_is_refused():
         with pytest.raises(pfast_amd.PfastError):
             ctx.set_check_interval(bad)
     ctx.close()
+
+
+def test_autotune_schedule_results_unchanged():
+    """The (schedule, check interval) tuner sets one of its candidates on the context and
+    the solve after it is bit-identical to the default's."""
+    import pfast_amd
+    c = cases.case3d(76, shape=(17, 26, 23), spacing=(25.0, 25.0, 25.0), nsrc=19, nrec_per=8)
+    ref = _solve(c)[0]
+    ctx = pfast_amd.Context(0)
+    m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+    sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+    chosen, ms = sh.autotune_schedule()
+    print(f"chosen {chosen}, times {ms}", flush=True)
+    assert chosen in pfast_amd.Shots3D.SCHEDULE_CANDIDATES and len(ms) == 5
+    assert all(t > 0 for t in ms.values())
+    sh.run(adjoint=True, MAXSwp=100)
+    g, E = sh.gradient()
+    out = (np.stack([sh.field(s) for s in range(len(c["src"]))]),
+           np.stack([sh.field(s, adjoint=True) for s in range(len(c["src"]))]), sh.cycles(), g, E)
+    sh.close()
+    m.close()
+    ctx.close()
+    _same(out, ref)
