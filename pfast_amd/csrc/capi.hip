@@ -1256,13 +1256,85 @@ void shots3d_free(pfast_shots3d *s)
 
 }  // namespace pfast
 
+namespace {
+int shots3d_create_ordered(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
+                           const int *rec, const double *nxyz, const double *T_obs,
+                           std::vector<int> perm, std::vector<int> iperm, std::vector<long> rmap,
+                           pfast_shots3d **out);
+}  // namespace
+
 extern "C" {
 
-int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
-                         const int *rec, const double *nxyz, const double *T_obs,
+int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src_in, const int *srnum_in,
+                         const int *rec_in, const double *nxyz_in, const double *T_obs_in,
                          pfast_shots3d **out)
 {
-    ARGCHK(m && src && srnum && out && nsrc > 0, "bad argument");
+    ARGCHK(m && src_in && srnum_in && out && nsrc > 0, "bad argument");
+    for (int i = 0; i < nsrc; i++) ARGCHK(srnum_in[i] >= 0, "negative srnum");
+    // internal source order: Morton order of the source nodes (stable), so that the
+    // chunks of C consecutive sources hold neighbours; the receiver rows move with
+    // their source
+    std::vector<int> perm(nsrc);
+    {
+        auto spread = [](unsigned v) {  // 21 bits -> every third bit of 63
+            unsigned long long x = v & 0x1FFFFFu;
+            x = (x | x << 32) & 0x1F00000000FFFFull;
+            x = (x | x << 16) & 0x1F0000FF0000FFull;
+            x = (x | x << 8) & 0x100F00F00F00F00Full;
+            x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+            x = (x | x << 2) & 0x1249249249249249ull;
+            return x;
+        };
+        std::vector<unsigned long long> key(nsrc);
+        for (int i = 0; i < nsrc; i++)
+            key[i] = spread((unsigned)std::max(src_in[3 * i], 0)) << 2 |
+                     spread((unsigned)std::max(src_in[3 * i + 1], 0)) << 1 |
+                     spread((unsigned)std::max(src_in[3 * i + 2], 0));
+        for (int i = 0; i < nsrc; i++) perm[i] = i;
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    std::vector<long> off_in(nsrc + 1, 0);
+    for (int i = 0; i < nsrc; i++) off_in[i + 1] = off_in[i] + srnum_in[i];
+    std::vector<int> src(3 * (size_t)nsrc), srnum(nsrc), rec, iperm(nsrc);
+    std::vector<double> nxyz, T_obs;
+    std::vector<long> rmap;
+    rmap.reserve(off_in[nsrc]);
+    for (int p = 0; p < nsrc; p++) {
+        const int u = perm[p];
+        iperm[u] = p;
+        for (int d = 0; d < 3; d++) src[3 * p + d] = src_in[3 * u + d];
+        srnum[p] = srnum_in[u];
+        for (long r = off_in[u]; r < off_in[u + 1]; r++) rmap.push_back(r);
+    }
+    if (rec_in) {
+        rec.resize(3 * rmap.size());
+        for (size_t r = 0; r < rmap.size(); r++)
+            for (int d = 0; d < 3; d++) rec[3 * r + d] = rec_in[3 * rmap[r] + d];
+    }
+    if (nxyz_in) {
+        nxyz.resize(3 * rmap.size());
+        for (size_t r = 0; r < rmap.size(); r++)
+            for (int d = 0; d < 3; d++) nxyz[3 * r + d] = nxyz_in[3 * rmap[r] + d];
+    }
+    if (T_obs_in) {
+        T_obs.resize(rmap.size());
+        for (size_t r = 0; r < rmap.size(); r++) T_obs[r] = T_obs_in[rmap[r]];
+    }
+    return shots3d_create_ordered(m, nsrc, src.data(), srnum.data(), rec_in ? rec.data() : nullptr,
+                                  nxyz_in ? nxyz.data() : nullptr, T_obs_in ? T_obs.data() : nullptr,
+                                  std::move(perm), std::move(iperm), std::move(rmap), out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// pfast_shots3d_create on inputs already in the internal source order.
+int shots3d_create_ordered(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
+                           const int *rec, const double *nxyz, const double *T_obs,
+                           std::vector<int> perm, std::vector<int> iperm, std::vector<long> rmap,
+                           pfast_shots3d **out)
+{
     const int nrows = m->nrows, ncols = m->ncols, nz = m->nz;
     const long N = (long)nrows * ncols * nz;
     // the sweep kernels address one chunk's field (C sources) with 32-bit byte offsets;
@@ -1274,10 +1346,12 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     ARGCHK(!(rec && T_obs && nxyz) || (N + 64L * nrows * ncols) * PFAST_CHUNK * 8 <= (1L << 32),
            "grid too large for the batched adjoint (one source chunk's (lambda, T) field plus "
            "64 levels must stay within 4 GiB)");
-    for (int i = 0; i < nsrc; i++) ARGCHK(srnum[i] >= 0, "negative srnum");
     auto *s = new pfast_shots3d();
     s->m = m;
     m->nshots++;  // undone by pfast_shots3d_destroy, also on the failure paths below
+    s->perm = std::move(perm);
+    s->iperm = std::move(iperm);
+    s->rmap = std::move(rmap);
     s->nsrc = nsrc;
     s->nchunk = (nsrc + PFAST_CHUNK - 1) / PFAST_CHUNK;
     s->src.assign(src, src + 3 * nsrc);
@@ -1391,6 +1465,10 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *
     *out = s;
     return PFAST_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pfast_shots3d_set_source_init(pfast_shots3d *s, int halfwidth)
 {
@@ -1540,8 +1618,9 @@ int pfast_shots3d_gradient(pfast_shots3d *s, double *grad, double *misfit)
     HIPCHK(hipMemcpyAsync(ee.data(), s->EE, s->nsrc * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (misfit) {
-        double sum = 0, comp = 0;  // Kahan over sources
-        for (double e : ee) {
+        double sum = 0, comp = 0;  // Kahan over sources, in the caller's order
+        for (int u = 0; u < s->nsrc; u++) {
+            const double e = ee[s->iperm[u]];
             const double y = e - comp, t = sum + y;
             comp = (t - sum) - y;
             sum = t;
@@ -1558,8 +1637,10 @@ int pfast_shots3d_picks(pfast_shots3d *s, double *picks)
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols * m->nz;
     HIPCHK(pfast::launch_gather_picks(st, s->T, s->rnode, s->rsrc, (int)s->nrec, N, s->picks));
-    HIPCHK(hipMemcpyAsync(picks, s->picks, s->nrec * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> p(s->nrec);
+    HIPCHK(hipMemcpyAsync(p.data(), s->picks, s->nrec * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    for (long r = 0; r < s->nrec; r++) picks[s->rmap[r]] = p[r];
     return PFAST_OK;
 }
 
@@ -1570,8 +1651,8 @@ int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field)
     pfast_model3d *m = s->m;
     hipStream_t st = m->ctx->stream;
     const long N = (long)m->nrows * m->ncols * m->nz;
-    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, src, N, which ? 2 : 1, which == 0,
-                                      s->fieldbuf));
+    HIPCHK(pfast::launch_gather_field(st, which ? s->L : s->T, s->iperm[src], N, which ? 2 : 1,
+                                      which == 0, s->fieldbuf));
     HIPCHK(hipMemcpyAsync(field, s->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return PFAST_OK;
@@ -1580,9 +1661,10 @@ int pfast_shots3d_field(pfast_shots3d *s, int which, int src, double *field)
 int pfast_shots3d_cycles(pfast_shots3d *s, int *cycles)
 {
     ARGCHK(s && cycles, "bad argument");
-    for (int i = 0; i < s->nsrc; i++) {
-        cycles[2 * i] = i < (int)s->fcyc.size() ? s->fcyc[i] : 0;
-        cycles[2 * i + 1] = i < (int)s->acyc.size() ? s->acyc[i] : 0;
+    for (int u = 0; u < s->nsrc; u++) {
+        const int i = s->iperm[u];
+        cycles[2 * u] = i < (int)s->fcyc.size() ? s->fcyc[i] : 0;
+        cycles[2 * u + 1] = i < (int)s->acyc.size() ? s->acyc[i] : 0;
     }
     return PFAST_OK;
 }
@@ -1594,6 +1676,7 @@ int pfast_shots3d_plain_source(pfast_shots3d *s, int srcid, int MAXSwp, double *
                                int *cycles2, double *T_out, double *lam_out)
 {
     ARGCHK(s && s->have_adj_data && srcid >= 0 && srcid < s->nsrc, "bad argument");
+    srcid = s->iperm[srcid];
     pfast_model3d *m = s->m;
     pfast_ctx *ctx = m->ctx;
     const long N = (long)m->nrows * m->ncols * m->nz;

@@ -960,9 +960,10 @@ int pfast_server_cycles(pfast_server *S, long ticket, int *cycles)
     if (!R) return PFAST_ERR_ARG;
     ARGCHK(R->state == PFAST_REQ_DONE, "request not done");
     if (R->s3) {
-        for (int i = 0; i < R->nsrc; i++) {
-            cycles[2 * i] = R->cyc_f[i];
-            cycles[2 * i + 1] = R->cyc_a.empty() ? 0 : R->cyc_a[i];
+        for (int u = 0; u < R->nsrc; u++) {  // the shot set's internal order -> the caller's
+            const int i = R->s3->iperm[u];
+            cycles[2 * u] = R->cyc_f[i];
+            cycles[2 * u + 1] = R->cyc_a.empty() ? 0 : R->cyc_a[i];
         }
         return PFAST_OK;
     }
@@ -990,7 +991,8 @@ int pfast_server_gradient(pfast_server *S, long ticket, double *grad, double *gr
         HIPCHK(hipStreamSynchronize(S->copy));
         if (misfit) {
             double sum = 0, comp = 0;  // Kahan over sources, as pfast_shots3d_gradient
-            for (double e : ee) {
+            for (int u = 0; u < R->nsrc; u++) {
+                const double e = ee[R->s3->iperm[u]];
                 const double y = e - comp, t = sum + y;
                 comp = (t - sum) - y;
                 sum = t;
@@ -1019,7 +1021,8 @@ int pfast_server_field(pfast_server *S, long ticket, int src, double *T)
     if (R->s3) {
         const pfast_model3d *m = R->s3->m;
         const long N = (long)m->nrows * m->ncols * m->nz;
-        HIPCHK(pfast::launch_gather_field(S->copy, R->s3->T, src, N, 1, 1, R->s3->fieldbuf));
+        HIPCHK(pfast::launch_gather_field(S->copy, R->s3->T, R->s3->iperm[src], N, 1, 1,
+                                          R->s3->fieldbuf));
         HIPCHK(hipMemcpyAsync(T, R->s3->fieldbuf, N * sizeof(double), hipMemcpyDeviceToHost, S->copy));
         HIPCHK(hipStreamSynchronize(S->copy));
         return PFAST_OK;

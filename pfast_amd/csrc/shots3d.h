@@ -23,6 +23,13 @@ struct pfast_shots3d {
     pfast_model3d *m;
     int nsrc, nchunk;
     long nrec;
+    // Sources are held in an internal order that puts spatially close sources in the
+    // same chunk (their sweeps change the same tiles in the same cycles, so more tiles
+    // are skipped and chunks converge together); every per-source table below is in
+    // that order.  perm[p] = the caller's index of internal source p, iperm its inverse;
+    // rmap[r] = the caller's receiver row of internal row r.
+    std::vector<int> perm, iperm;
+    std::vector<long> rmap;
     std::vector<int> src;    // 3*nsrc
     std::vector<int> srnum;  // nsrc
     std::vector<long> ch0;   // nsrc+1 prefix
