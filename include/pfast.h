@@ -246,6 +246,25 @@ int pfast_ctx_set_check_interval(pfast_ctx *ctx, int cycles);
  * kept per shot set (at most 16), then launched as one graph.  Off by default and
  * ignored while timing is on; results are the same. */
 int pfast_ctx_set_graphs(pfast_ctx *ctx, int enable);
+/* Internal source order of the batched 3D shot sets created afterwards on this context
+ * (no counterpart in the reference, which runs one source per MPI rank), see
+ * pfast_source_order: 2 (default) = balanced clusters, 1 = Morton order, 0 = the
+ * caller's order (group the sources yourself).  Eight consecutive sources share a
+ * sweep; neighbours' sweeps change the same tiles in the same cycles, so more tiles
+ * are skipped and chunks converge together.  Every per-source result is returned in the
+ * caller's order, and each source's fields and cycle counts are the same bit for bit
+ * under any order. */
+int pfast_ctx_set_source_order(pfast_ctx *ctx, int mode);
+/* The internal order itself (host only, no GPU needed): perm[p] = the caller's index of
+ * internal source p, for nsrc source nodes src[3*i] = (row, column, depth) on a grid of
+ * spacings dx (columns), dy (rows), dz (depth).  mode 0: identity; 1: Morton order of the
+ * nodes (stable); 2: clusters of 8 sources (the last holds the remainder) minimising the
+ * distance to their centroid, by Lloyd iterations from the Morton chunks with a greedy
+ * capacity-bounded assignment (nearest pairs first, each source among its 16 nearest
+ * centroids), clusters and members in Morton order; falls back to 1 when nsrc x clusters
+ * exceeds 1e8.  Deterministic. */
+int pfast_source_order(int nsrc, const int *src, double dx, double dy, double dz, int mode,
+                       int *perm);
 
 /* ---------------------------------------------------------------------------------
  * Batched 2D shot sets: every source of a 2D survey resident on one GPU (FP32

@@ -189,6 +189,13 @@ class Context:
         results and cycle counts do not change)."""
         check(lib().pfast_ctx_set_check_interval(self.h, int(cycles)))
 
+    def set_source_order(self, mode="clusters"):
+        """Internal source order of 3D shot sets created afterwards: "clusters" (default;
+        balanced clusters of eight neighbouring sources), "morton" or "caller".  Results
+        come back in the caller's order either way."""
+        m = {"caller": 0, "morton": 1, "clusters": 2}[mode] if isinstance(mode, str) else int(mode)
+        check(lib().pfast_ctx_set_source_order(self.h, m))
+
     def set_graphs(self, enable=True):
         """Replay the cycles between two checks from captured hipGraphs."""
         check(lib().pfast_ctx_set_graphs(self.h, 1 if enable else 0))
@@ -209,6 +216,19 @@ class Context:
         check(lib().pfast_ctx_work(self.h, int(kind), C.byref(w)))
         check(lib().pfast_ctx_work_swept(self.h, int(kind), C.byref(sw)))
         return ms.value, n.value, w.value, sw.value
+
+
+def source_order(src, spacing=(1.0, 1.0, 1.0), mode="clusters"):
+    """The internal order a 3D shot set gives its sources (pfast_source_order): perm[p] =
+    the caller's index of internal source p; src rows are (row, column, depth) nodes,
+    spacing (dx, dy, dz).  Host only."""
+    s = _i32(src).reshape(-1, 3)
+    m = {"caller": 0, "morton": 1, "clusters": 2}[mode] if isinstance(mode, str) else int(mode)
+    flat = np.ascontiguousarray(s.ravel())
+    perm = np.zeros(len(s), np.int32)
+    check(lib().pfast_source_order(len(s), ptr(flat), float(spacing[0]), float(spacing[1]),
+                                   float(spacing[2]), m, ptr(perm)))
+    return perm
 
 
 def sweep_variants():

@@ -65,6 +65,8 @@ _SIGS = {
     "pfast_ctx_set_dataflow": [vpp, i],
     "pfast_ctx_set_check_interval": [vpp, i],
     "pfast_ctx_set_graphs": [vpp, i],
+    "pfast_ctx_set_source_order": [vpp, i],
+    "pfast_source_order": [i, vpp, C.c_double, C.c_double, C.c_double, i, vpp],
     "pfast_model3d_create": [vpp, dp, i, i, i, d, d, d, C.POINTER(vpp)],
     "pfast_model3d_set_vp": [vpp, dp],
     "pfast_model3d_destroy": [vpp],

@@ -247,6 +247,14 @@ int pfast_ctx_set_check_interval(pfast_ctx *ctx, int cycles)
     return PFAST_OK;
 }
 
+int pfast_ctx_set_source_order(pfast_ctx *ctx, int mode)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ARGCHK(mode >= 0 && mode <= 2, "source order must be 0, 1 or 2");
+    ctx->source_order = mode;
+    return PFAST_OK;
+}
+
 int pfast_ctx_set_graphs(pfast_ctx *ctx, int enable)
 {
     ARGCHK(ctx, "ctx is NULL");
@@ -1256,6 +1264,114 @@ void shots3d_free(pfast_shots3d *s)
 
 }  // namespace pfast
 
+extern "C" int pfast_source_order(int nsrc, const int *src, double dx, double dy, double dz,
+                                  int mode, int *perm)
+{
+    ARGCHK(nsrc >= 0 && (nsrc == 0 || (src && perm)), "bad argument");
+    ARGCHK(mode >= 0 && mode <= 2, "source order must be 0, 1 or 2");
+    for (int i = 0; i < nsrc; i++) perm[i] = i;
+    if (mode == 0 || nsrc <= 1) return PFAST_OK;
+    // Morton key of the source node (row, column, depth), 21 bits each
+    auto spread = [](unsigned v) {  // bit b of v -> bit 3b
+        unsigned long long x = v & 0x1FFFFFu;
+        x = (x | x << 32) & 0x1F00000000FFFFull;
+        x = (x | x << 16) & 0x1F0000FF0000FFull;
+        x = (x | x << 8) & 0x100F00F00F00F00Full;
+        x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    std::vector<unsigned long long> key(nsrc);
+    for (int i = 0; i < nsrc; i++)
+        key[i] = spread((unsigned)std::max(src[3 * i], 0)) << 2 |
+                 spread((unsigned)std::max(src[3 * i + 1], 0)) << 1 |
+                 spread((unsigned)std::max(src[3 * i + 2], 0));
+    std::stable_sort(perm, perm + nsrc, [&](int a, int b) { return key[a] < key[b]; });
+    const int C = PFAST_CHUNK, k = (nsrc + C - 1) / C;
+    if (mode == 1 || k < 2 || (double)nsrc * k > 1e8) return PFAST_OK;
+    // mode 2: balanced clusters of C sources (the last one holds the remainder), by
+    // Lloyd iterations from the Morton chunks with a greedy capacity-bounded assignment
+    // (nearest pairs first); deterministic
+    std::vector<double> P(3 * (size_t)nsrc), cen(3 * (size_t)k);
+    for (int i = 0; i < nsrc; i++) {
+        P[3 * i] = src[3 * i] * dy;
+        P[3 * i + 1] = src[3 * i + 1] * dx;
+        P[3 * i + 2] = src[3 * i + 2] * dz;
+    }
+    std::vector<int> lab(nsrc);
+    for (int p = 0; p < nsrc; p++) lab[perm[p]] = p / C;
+    auto centroids = [&]() {
+        std::vector<int> n(k, 0);
+        std::fill(cen.begin(), cen.end(), 0.0);
+        for (int i = 0; i < nsrc; i++) {
+            n[lab[i]]++;
+            for (int d = 0; d < 3; d++) cen[3 * lab[i] + d] += P[3 * i + d];
+        }
+        for (int c = 0; c < k; c++)
+            for (int d = 0; d < 3; d++) cen[3 * c + d] /= std::max(n[c], 1);
+    };
+    // each source competes for its M nearest centroids; one left without a place
+    // (rare) takes the nearest cluster with room
+    const int M = std::min(k, 16);
+    std::vector<std::pair<double, long>> pairs, near((size_t)k);
+    for (int it = 0; it < 12; it++) {
+        centroids();
+        pairs.clear();
+        for (int i = 0; i < nsrc; i++) {
+            for (int c = 0; c < k; c++) {
+                double d2 = 0;
+                for (int d = 0; d < 3; d++) {
+                    const double t = P[3 * i + d] - cen[3 * c + d];
+                    d2 += t * t;
+                }
+                near[c] = {d2, (long)i * k + c};
+            }
+            std::nth_element(near.begin(), near.begin() + (M - 1), near.end());
+            pairs.insert(pairs.end(), near.begin(), near.begin() + M);
+        }
+        std::sort(pairs.begin(), pairs.end());
+        std::vector<int> cap(k, C), nl(nsrc, -1);
+        cap[k - 1] = nsrc - (k - 1) * C;
+        for (const auto &q : pairs) {
+            const int i = (int)(q.second / k), c = (int)(q.second % k);
+            if (nl[i] < 0 && cap[c] > 0) {
+                nl[i] = c;
+                cap[c]--;
+            }
+        }
+        for (int i = 0; i < nsrc; i++) {
+            if (nl[i] >= 0) continue;
+            double best = 0;
+            for (int c = 0; c < k; c++) {
+                if (!cap[c]) continue;
+                double d2 = 0;
+                for (int d = 0; d < 3; d++) {
+                    const double t = P[3 * i + d] - cen[3 * c + d];
+                    d2 += t * t;
+                }
+                if (nl[i] < 0 || d2 < best) {
+                    best = d2;
+                    nl[i] = c;
+                }
+            }
+            cap[nl[i]]--;
+        }
+        const bool same = nl == lab;
+        lab.swap(nl);
+        if (same) break;
+    }
+    // clusters in Morton order of their first member, members in Morton order
+    std::vector<int> first(k, nsrc);
+    std::vector<int> rank(nsrc);
+    for (int p = 0; p < nsrc; p++) rank[perm[p]] = p;
+    for (int i = 0; i < nsrc; i++) first[lab[i]] = std::min(first[lab[i]], rank[i]);
+    std::stable_sort(perm, perm + nsrc, [&](int a, int b) {
+        const int fa = first[lab[a]], fb = first[lab[b]];
+        return fa != fb ? fa < fb : rank[a] < rank[b];
+    });
+    return PFAST_OK;
+}
+
 namespace {
 int shots3d_create_ordered(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
                            const int *rec, const double *nxyz, const double *T_obs,
@@ -1271,28 +1387,10 @@ int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src_in, const in
 {
     ARGCHK(m && src_in && srnum_in && out && nsrc > 0, "bad argument");
     for (int i = 0; i < nsrc; i++) ARGCHK(srnum_in[i] >= 0, "negative srnum");
-    // internal source order: Morton order of the source nodes (stable), so that the
-    // chunks of C consecutive sources hold neighbours; the receiver rows move with
-    // their source
+    // internal source order: chunks of spatially close sources (the receiver rows move
+    // with their source)
     std::vector<int> perm(nsrc);
-    {
-        auto spread = [](unsigned v) {  // 21 bits -> every third bit of 63
-            unsigned long long x = v & 0x1FFFFFu;
-            x = (x | x << 32) & 0x1F00000000FFFFull;
-            x = (x | x << 16) & 0x1F0000FF0000FFull;
-            x = (x | x << 8) & 0x100F00F00F00F00Full;
-            x = (x | x << 4) & 0x10C30C30C30C30C3ull;
-            x = (x | x << 2) & 0x1249249249249249ull;
-            return x;
-        };
-        std::vector<unsigned long long> key(nsrc);
-        for (int i = 0; i < nsrc; i++)
-            key[i] = spread((unsigned)std::max(src_in[3 * i], 0)) << 2 |
-                     spread((unsigned)std::max(src_in[3 * i + 1], 0)) << 1 |
-                     spread((unsigned)std::max(src_in[3 * i + 2], 0));
-        for (int i = 0; i < nsrc; i++) perm[i] = i;
-        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
-    }
+    pfast_source_order(nsrc, src_in, m->dx, m->dy, m->dz, m->ctx->source_order, perm.data());
     std::vector<long> off_in(nsrc + 1, 0);
     for (int i = 0; i < nsrc; i++) off_in[i + 1] = off_in[i] + srnum_in[i];
     std::vector<int> src(3 * (size_t)nsrc), srnum(nsrc), rec, iperm(nsrc);

@@ -51,6 +51,10 @@ struct pfast_ctx {
     // whether those cycles are replayed from a captured hipGraph
     int check_every = 1;
     int graphs = 0;
+    // internal source order of new batched 3D shot sets (pfast_source_order): 0 the
+    // caller's, 1 Morton order of the source nodes, 2 balanced clusters (chunks of
+    // neighbours)
+    int source_order = 2;
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> fork_join;  // [0] fork on stream, [1..] joins on aux
 };
