@@ -550,7 +550,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // marks a fixed receiver node), and a changed node stores its lambda alone.  Record
 // offsets are twice the traveltime field's.  Only interior nodes are updated, and
 // every neighbour they read is inside the domain.
-template <int TI, int TJ, int C, int L, bool S3POS, bool FLOW>
+template <int TI, int TJ, int C, int L, bool UNI, bool S3POS, bool FLOW>
 __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
                                          int six, float2 *pub, int *any_change, FlowWait *fw,
                                          unsigned *self)
@@ -668,17 +668,22 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             const float tc = vcur.y;
             const float txm = fabsf(xm.y), txp = fabsf(xp.y), tym = fabsf(ym.y),
                         typ = fabsf(yp.y), tzm = fabsf(zm.y), tzp = fabsf(zp.y);
-            const float I = PFAST_INF_F;
-            const bool anyinf = tc >= I || txm >= I || txp >= I || tym >= I || typ >= I ||
-                                tzm >= I || tzp >= I;
-            float den = 0.0f;
-            float num = adj_term(tc, txp, xp.x, wx, den);
-            num += adj_term(tc, txm, xm.x, wx, den);
-            num += adj_term(tc, typ, yp.x, wy, den);
-            num += adj_term(tc, tym, ym.x, wy, den);
-            num += adj_term(tc, tzp, zp.x, wz, den);
-            num += adj_term(tc, tzm, zm.x, wz, den);
-            const float nw = !anyinf && fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+            // any stencil traveltime at INF: one compare of the maximum (three max3)
+            const float tmax = fmaxf(fmaxf(fmaxf(txm, txp), tym),
+                                     fmaxf(fmaxf(typ, tzm), fmaxf(tzp, tc)));
+            float num = 0.0f, den = 0.0f;
+            adj_acc<UNI>(txp - tc, xp.x, wx, num, den);
+            adj_acc<UNI>(txm - tc, xm.x, wx, num, den);
+            adj_acc<UNI>(typ - tc, yp.x, wy, num, den);
+            adj_acc<UNI>(tym - tc, ym.x, wy, num, den);
+            adj_acc<UNI>(tzp - tc, zp.x, wz, num, den);
+            adj_acc<UNI>(tzm - tc, zm.x, wz, num, den);
+            // the reference's threshold applies to the weighted denominator
+            const float denw = UNI ? den * wx : den;
+            // computed in every lane and selected: no branch around the update
+            float r = num * frcp(den);
+            asm volatile("" : "+v"(r));
+            const float nw = tmax < PFAST_INF_F && denw > 1e-9f ? r : 0.0f;
             const bool act = p >= plo && p < phi && !signbit(vcur.y);
             const bool c = act && nw != vcur.x;
             const float nl = act ? nw : vcur.x;
@@ -714,7 +719,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
                 any_change);
 }
 
-template <int TI, int TJ, int C, int L, bool S3POS>
+template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
 {
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
@@ -725,7 +730,7 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    adj_tile<TI, TJ, C, L, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr, nullptr);
+    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr, nullptr);
 }
 
 // ======================================================= dataflow kernels
@@ -905,7 +910,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_finish(f, self);
 }
 
-template <int TI, int TJ, int C, int L, bool S3POS>
+template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void adj3d_flow(SweepArgs a, FlowArgs f)
 {
     flow_phase(f);
@@ -923,7 +928,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_task<TI, TJ>(a, f, s, t, at, fw, self);
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
-    adj_tile<TI, TJ, C, L, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
+    adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
                                         &any_change, &fw, self);
     flow_finish(f, self);
 }
@@ -1196,13 +1201,14 @@ namespace pfast {
 
 // Sweep kernel variants (tile rows x columns, batch depth of the forward / adjoint
 // kernel), all of the column-window schedule.  Variant 0 is the default; the others
-// stay selectable for measurement.  The adjoint keeps a (lambda, T) pair per queued
-// value, so a shallower batch keeps its register count at full occupancy (8x8 tiles:
-// 60 VGPRs at L2 against 86 at L4, 8 against 5 waves per SIMD).
+// stay selectable for measurement and for the autotuner.  The adjoint keeps a
+// (lambda, T) pair per queued value, so a shallower batch keeps its register count at
+// full occupancy (8x8 tiles: 54 VGPRs at L2 against 84 at L4, 8 against 5 waves per
+// SIMD).  Tiles of 1024 threads (8x16, 16x8) and batch depth 8 measured 1.4-2.8x slower
+// on the 400-source workload and are not built.
 #define PFAST_VARIANTS(X)                                                                 \
-    X(8, 8, 4, 2) X(8, 8, 4, 4) X(8, 8, 6, 6) X(8, 8, 2, 2) X(8, 8, 8, 8) X(8, 8, 6, 2)     \
-    X(8, 4, 6, 6) X(8, 4, 4, 4) X(4, 8, 6, 6) X(4, 4, 6, 6) X(4, 4, 4, 4) X(8, 16, 6, 6)   \
-    X(8, 16, 4, 4) X(16, 8, 6, 6) X(4, 16, 6, 6)
+    X(8, 8, 4, 2) X(8, 8, 4, 4) X(8, 8, 6, 6) X(8, 8, 2, 2) X(8, 8, 6, 2) X(8, 4, 4, 4)     \
+    X(4, 8, 6, 6) X(4, 4, 4, 4) X(4, 16, 6, 6)
 #define PFAST_VARIANT_ENTRY(TI, TJ, L, LA) \
     {0, TI, TJ, L, LA, "column-window " #TI "x" #TJ " L" #L "/" #LA},
 static const SweepVariant kVariants[] = {PFAST_VARIANTS(PFAST_VARIANT_ENTRY)};
@@ -1221,10 +1227,17 @@ void launch_col(hipStream_t st, const SweepArgs &a, int adjoint, int uniform, in
     static_assert(TI * TJ * C <= 1024, "one thread per column and source");
     const dim3 block(TI * TJ * C);
     if (adjoint) {
-        if (s3 > 0)
-            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, true>), grid, block, 0, st, a, D, s1, s2);
-        else
-            hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, false>), grid, block, 0, st, a, D, s1, s2);
+        if (uniform) {
+            if (s3 > 0)
+                hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, true, true>), grid, block, 0, st, a, D, s1, s2);
+            else
+                hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, true, false>), grid, block, 0, st, a, D, s1, s2);
+        } else {
+            if (s3 > 0)
+                hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, false, true>), grid, block, 0, st, a, D, s1, s2);
+            else
+                hipLaunchKernelGGL((adj3d_col<TI, TJ, C, LA, false, false>), grid, block, 0, st, a, D, s1, s2);
+        }
     } else if (uniform) {
         hipLaunchKernelGGL((fwd3d_col<TI, TJ, C, L, true>), grid, block, 0, st, a, D, s1, s2, s3);
     } else {
@@ -1253,7 +1266,9 @@ template <int TI, int TJ, int L, int LA, bool S3POS>
 const void *flow_kernel(int adjoint, int uniform)
 {
     constexpr int C = PFAST_CHUNK;
-    if (adjoint) return reinterpret_cast<const void *>(&adj3d_flow<TI, TJ, C, LA, S3POS>);
+    if (adjoint)
+        return uniform ? reinterpret_cast<const void *>(&adj3d_flow<TI, TJ, C, LA, true, S3POS>)
+                       : reinterpret_cast<const void *>(&adj3d_flow<TI, TJ, C, LA, false, S3POS>);
     if (uniform) return reinterpret_cast<const void *>(&fwd3d_flow<TI, TJ, C, L, true, S3POS>);
     return reinterpret_cast<const void *>(&fwd3d_flow<TI, TJ, C, L, false, S3POS>);
 }
@@ -1287,7 +1302,7 @@ int flow_grid(int variant, int adjoint, int uniform)
     static std::map<long, int> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    const int kind = adjoint ? 2 : uniform ? 1 : 0;
+    const int kind = (adjoint ? 2 : 0) + (uniform ? 1 : 0);
     const long key = ((long)dev * 64 + variant) * 4 + kind;
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);

@@ -120,11 +120,13 @@ __device__ __forceinline__ float eik3d_f32_uniform(float a, float b, float c, fl
     const float mid = __builtin_amdgcn_fmed3f(a, b, c);
     const float q1 = lo + f;
     const float d = mid - lo;
-    const float q2 = mid + 0.5f * (fsqrt(fmaf(-d, d, 2.0f * f * f)) - d);
+    const float ff = f * f;
+    // ff + ff is 2 f f rounded once, as (2 f) f would be (scaling by 2 is exact)
+    const float q2 = fmaf(0.5f, fsqrt(fmaf(-d, d, ff + ff)) - d, mid);
     const float db = hi - mid, dc = hi - lo;
     const float P = db + dc;
-    const float disc3 = fmaf(P, P, -3.0f * (fmaf(db, db, dc * dc) - f * f));
-    const float q3 = disc3 >= 0.0f ? hi + (fsqrt(disc3) - P) * (1.0f / 3.0f) : PFAST_INF_F;
+    const float disc3 = fmaf(P, P, -3.0f * (fmaf(db, db, dc * dc) - ff));
+    const float q3 = disc3 >= 0.0f ? fmaf(fsqrt(disc3) - P, 1.0f / 3.0f, hi) : PFAST_INF_F;
     return q1 <= mid ? q1 : (q2 <= hi ? q2 : q3);
 }
 
@@ -178,4 +180,22 @@ __device__ __forceinline__ float adj_term(float tc, float tn, float ln, float w,
 {
     den = fmaf(w, fmaxf(tc - tn, 0.0f), den);
     return tn > tc ? w * (tn - tc) * ln : 0.0f;
+}
+
+// Batched 3D adjoint accumulation from the difference d = tn - tc of one neighbour:
+// den += w max(-d, 0), num += w max(d, 0) ln, the product added by one fma.  num and
+// den start at +0: a downwind neighbour's product is a zero whose sign is that of ln,
+// and a signed zero added to +0 (or to a non-zero sum) leaves it unchanged, so the
+// result never depends on downwind lambdas, not even through the sign of a zero.
+// Uniform spacing (UNI): every w is the same and cancels in num / den; it is left out.
+template <bool UNI>
+__device__ __forceinline__ void adj_acc(float d, float ln, float w, float &num, float &den)
+{
+    if (UNI) {
+        den += fmaxf(-d, 0.0f);
+        num = fmaf(fmaxf(d, 0.0f), ln, num);
+    } else {
+        den = fmaf(w, fmaxf(-d, 0.0f), den);
+        num = fmaf(w * fmaxf(d, 0.0f), ln, num);
+    }
 }
