@@ -120,7 +120,7 @@ struct ColLane {
     bool valid;
     bool uxm, uxp, uym, uyp;   // that neighbour is a halo node (read from HBM)
     int rxm, rxp, rym, ryp;    // LDS indices of the neighbours' published values
-    unsigned oI, oHX, oHY;     // byte offsets of level 0: own column, x halo, y halo
+    unsigned oI;               // byte offset of level 0 of the own column
     unsigned oF;               // byte offset of level 0 in the cell-time array
     unsigned step, stepF;      // byte step per level (two's complement when s3 < 0)
     unsigned node;
@@ -156,14 +156,8 @@ __device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SweepArgs &a, 
     l.ryp = !ok ? tid : eyp ? INFSLOT : (s1 > 0 ? NT : 0) + tid + C * TJ;
     const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
     l.node = l.valid ? (unsigned)gi * a.ncols + gj : 0u;
-    const int hxj = l.uxm ? t.j0 - 1 : t.j0 + t.TJe;
-    const int hyi = l.uym ? t.i0 - 1 : t.i0 + t.TIe;
-    const bool hx = l.uxm || l.uxp, hy = l.uym || l.uyp;
     const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
-    const unsigned NONE = 0x80000000u;  // out of range at every level (never loaded)
     l.oI = 4u * (k0 * pC + l.node * C + l.c);
-    l.oHX = hx ? 4u * (k0 * pC + ((unsigned)gi * a.ncols + hxj) * C + l.c) : NONE;
-    l.oHY = hy ? 4u * (k0 * pC + ((unsigned)hyi * a.ncols + gj) * C + l.c) : NONE;
     l.oF = 4u * (k0 * plane + l.node);
     l.step = s3 > 0 ? 4u * pC : 0u - 4u * pC;
     l.stepF = s3 > 0 ? 4u * plane : 0u - 4u * plane;
@@ -340,14 +334,81 @@ __device__ __forceinline__ void fstore(rsrc_t r, unsigned off, float v)
     else bstore(r, off, v);
 }
 
+// ------------------------------------------------------------ halo staging
+// A tile's four halo faces (the columns beside its x edges, the rows beside its y
+// edges) are read by the lanes on those edges only: with a load per lane, a wave of
+// an edge row would carry a second full row of loads and every wave a sliver of x
+// halo per step, and the block waits at each step for its slowest wave.  Instead
+// every thread of the block loads a share of the faces for a whole batch of L steps
+// (full-wave loads, consecutive lanes on consecutive sources of one face node), a
+// batch ahead like the column's own values; at the batch start it writes them into an
+// LDS stage, and after the first step's barrier each edge lane reads its L values from
+// there.  Stage slot of step u, face f, position q along the face, source c:
+//   u * HSU + face offset + q * C + c,  faces x- (TI rows), x+ (TI), y- (TJ columns), y+ (TJ).
+// A value stands for the level the consuming edge lane works on at that step, so the
+// loader takes the consumer's sweep position into its address.
+template <int TI, int TJ, int C, int L>
+struct HaloStage {
+    static constexpr int NT = TI * TJ * C;
+    static constexpr int HSU = 2 * (TI + TJ) * C;  // stage slots per step
+    static constexpr int NS = L * HSU;             // per batch (one parity)
+    static constexpr int K = (NS + NT - 1) / NT;   // slots loaded per thread
+    unsigned addr[K];  // byte offset of the slot's value at batch start step 0
+    bool use[K];       // the slot feeds an edge lane that reads a halo in the domain
+    int xs, ys;        // this lane's x / y face slot at step 0 (edge lanes)
+};
+
+template <int TI, int TJ, int C, int L>
+__device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepArgs &a,
+                                                                   const TileCtx &t,
+                                                                   const ColLane<TI, TJ, C> &l,
+                                                                   int s1, int s2, int s3,
+                                                                   unsigned rec)
+{
+    using H = HaloStage<TI, TJ, C, L>;
+    H h;
+    const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
+    const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
+    const unsigned st = (s3 > 0 ? 4u * pC : 0u - 4u * pC) * rec;
+    const bool hxm = t.j0 > 0, hxp = t.j0 + t.TJe < a.ncols;
+    const bool hym = t.i0 > 0, hyp = t.i0 + t.TIe < a.nrows;
+#pragma unroll
+    for (int k = 0; k < H::K; k++) {
+        const int s = (int)threadIdx.x + k * H::NT;
+        const int u = s / H::HSU, r = s % H::HSU;
+        const int c = r % C, f = r / C;  // f: face position index over all four faces
+        int li, lj, gi, gj;
+        bool in;
+        if (f < TI) {                      // x-: row f, read at column j0 - 1
+            li = f; lj = 0; gi = t.i0 + f; gj = t.j0 - 1; in = hxm && f < t.TIe;
+        } else if (f < 2 * TI) {           // x+: row f - TI, column j0 + TJe
+            li = f - TI; lj = t.TJe - 1; gi = t.i0 + li; gj = t.j0 + t.TJe; in = hxp && li < t.TIe;
+        } else if (f < 2 * TI + TJ) {      // y-: column f - 2TI, row i0 - 1
+            li = 0; lj = f - 2 * TI; gi = t.i0 - 1; gj = t.j0 + lj; in = hym && lj < t.TJe;
+        } else {                           // y+: column, row i0 + TIe
+            li = t.TIe - 1; lj = f - 2 * TI - TJ; gi = t.i0 + t.TIe; gj = t.j0 + lj;
+            in = hyp && lj < t.TJe;
+        }
+        in = in && s < H::NS;
+        const int pipj = (s1 > 0 ? li : t.TIe - 1 - li) + (s2 > 0 ? lj : t.TJe - 1 - lj);
+        const unsigned node = in ? (unsigned)gi * a.ncols + (unsigned)gj : 0u;
+        h.use[k] = in;
+        h.addr[k] = 4u * rec * (k0 * pC + node * C + (unsigned)c) + (unsigned)(u - pipj) * st;
+    }
+    const int lc = l.c;
+    h.xs = (l.uxm ? 0 : TI * C) + l.li * C + lc;
+    h.ys = (l.uym ? 2 * TI * C : (2 * TI + TJ) * C) + l.lj * C + lc;
+    return h;
+}
+
 // ======================================================= column-window tiles
 // One tile sweep of the forward column window (the body of fwd3d_col and of each
 // task of fwd3d_flow).  FLOW: wait on the task's dependencies before each batch's
 // memory operations and publish the task's progress every batch (flow.h).
 template <int TI, int TJ, int C, int L, bool UNI, bool FLOW>
 __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int s3, int six, float *pub, int *any_change,
-                                         FlowWait *fw, unsigned *self)
+                                         int s3, int six, float *pub, float *hstage,
+                                         int *any_change, FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
@@ -397,17 +458,18 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
         oF = 4u * (k0 * tplane + tix * 64u + (unsigned)(l.li * TJ + l.lj));
         stF = s3 > 0 ? 4u * tplane : 0u - 4u * tplane;
     }
-    // The vector-memory path (L1 misses above all) bounds the sweep's throughput,
-    // so: halo loads run only in the lanes that use them (a wave without such a lane
-    // skips them), and the cell time, one value per node, is loaded for four levels
-    // at once, lane c of each DPP quad of the node's lanes taking level 4g + c%4;
-    // step u broadcasts it within the quad.
+    // The vector-memory path bounds the sweep's throughput, so: the halo faces are
+    // loaded by the whole block into an LDS stage (HaloStage), and the cell time, one
+    // value per node, is loaded for four levels at once, lane c of each DPP quad of the
+    // node's lanes taking level 4g + c%4; step u broadcasts it within the quad.
     constexpr int FQ = (L + 3) / 4;
-    const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
+    using HS = HaloStage<TI, TJ, C, L>;
+    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, l, s1, s2, s3, 1u);
     const unsigned oFc = oF + (unsigned)(l.c & 3) * stF;
+    float hl[HS::K];  // this thread's stage values of the next batch
     // queue of one batch: own column two levels ahead (m+2, INF outside the column),
-    // halos at level m itself, cell time of levels m .. m+4*FQ-1
-    auto load_batch = [&](int pb, float *ow, float *hx, float *hy, float *fv) {
+    // cell time of levels m .. m+4*FQ-1; and the stage values of that batch
+    auto load_batch = [&](int pb, float *ow, float *fv) {
         const int mb = pb - l.pipj;
         const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
 #pragma unroll
@@ -415,20 +477,9 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             const float o = fload<FLOW>(RF, l.oI + dm + (unsigned)(u + 2) * st);
             ow[u] = (unsigned)(mb + u + 2) < (unsigned)nz ? o : INF;
         }
-        if (hxuse) {
+        const unsigned dp = (unsigned)pb * st;
 #pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = fload<FLOW>(RF, l.oHX + dm + (unsigned)u * st);
-        } else {
-#pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = INF;
-        }
-        if (hyuse) {
-#pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = fload<FLOW>(RF, l.oHY + dm + (unsigned)u * st);
-        } else {
-#pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = INF;
-        }
+        for (int k = 0; k < HS::K; k++) hl[k] = hs.use[k] ? fload<FLOW>(RF, hs.addr[k] + dp) : INF;
 #pragma unroll
         for (int g = 0; g < FQ; g++) fv[g] = bload(RS, oFc + dmf + (unsigned)(4 * g) * stF);
     };
@@ -441,27 +492,31 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     float vprev = INF;
     float vcur = l.valid && m0 == 0 ? fload<FLOW>(RF, l.oI) : INF;
     float vnext = l.valid && m0 >= -1 ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
-    float qa[4][L], qb[4][L];
-    load_batch(0, qa[0], qa[1], qa[2], qa[3]);
+    float qa[2][L], qb[2][L];
+    load_batch(0, qa[0], qa[1]);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();  // INF slot visible
 
     bool changed = false;
-    float sval[L];
+    float sval[L], hx[L], hy[L];
     bool chg[L];
 #pragma unroll
     for (int u = 0; u < L; u++) {
         sval[u] = 0.0f;
         chg[u] = false;
     }
-    // one batch of L steps from queue q (loaded a batch earlier); issues the loads
-    // of the following batch into nq first
-    auto batch = [&](int pb, float (*q)[L], float (*nq)[L]) {
+    // one batch of L steps from queue q (loaded a batch earlier) and stage parity par;
+    // issues the loads of the following batch into nq first
+    auto batch = [&](int pb, int par, float (*q)[L], float (*nq)[L]) {
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
             flow_prefetch(*fw);
         }
+        float *Hs = hstage + par * HS::NS;
+#pragma unroll
+        for (int k = 0; k < HS::K; k++)
+            if (HS::NS % HS::NT == 0 || tid + k * HS::NT < HS::NS) Hs[tid + k * HS::NT] = hl[k];
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
@@ -473,7 +528,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             for (int u = 0; u < L; u++)
                 if (chg[u]) fstore<FLOW>(RF, l.oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pbo + L, nq[0], nq[1], nq[2], nq[3]);
+        load_batch(pbo + L, nq[0], nq[1]);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
@@ -485,11 +540,18 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
                 // every wave has waited for its stores of steps < pb - L
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
             }
+            if (u == 0) {  // the batch's halo values of this lane's faces
+#pragma unroll
+                for (int v = 0; v < L; v++) {
+                    hx[v] = Hs[hs.xs + v * HS::HSU];
+                    hy[v] = Hs[hs.ys + v * HS::HSU];
+                }
+            }
             const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            const float xa = fminf(l.uxm ? q[1][u] : pxm, l.uxp ? q[1][u] : pxp);
-            const float ya = fminf(l.uym ? q[2][u] : pym, l.uyp ? q[2][u] : pyp);
+            const float xa = fminf(l.uxm ? hx[u] : pxm, l.uxp ? hx[u] : pxp);
+            const float ya = fminf(l.uym ? hy[u] : pym, l.uyp ? hy[u] : pyp);
             const float za = fminf(vprev, vnext);
-            const float fval = quad_bcast(q[3][u >> 2], u);
+            const float fval = quad_bcast(q[1][u >> 2], u);
             float nv = vcur;
             bool c = false;
             if ((unsigned)(p - plo) < (unsigned)nz && (unsigned)(p - pfl) > pfn) {
@@ -507,13 +569,13 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             vnext = q[0][u];
         }
     };
-    // steps 0 .. pend-1, in batches of L (the queues alternate); the loop ends after
-    // the batch holding step pend-1 (uniform over the block)
+    // steps 0 .. pend-1, in batches of L (the queues and stage parities alternate); the
+    // loop ends after the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
     for (int pb = 0;; pb += 2 * L) {
-        batch(pb, qa, qb);
+        batch(pb, 0, qa, qb);
         if (pb + L >= pend) break;
-        batch(pb + L, qb, qa);
+        batch(pb + L, 1, qb, qa);
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -536,13 +598,15 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 {
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
+    __shared__ float hstage[2 * HaloStage<TI, TJ, C, L>::NS];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr, nullptr);
+    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, hstage, &any_change, nullptr,
+                                       nullptr);
 }
 
 // Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
@@ -552,8 +616,8 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS, bool FLOW>
 __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int six, float2 *pub, int *any_change, FlowWait *fw,
-                                         unsigned *self)
+                                         int six, float2 *pub, float2 *hstage, int *any_change,
+                                         FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     constexpr int s3 = S3POS ? 1 : -1;
@@ -568,7 +632,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     // 8-byte records: the traveltime field's offsets doubled, mod 2^32 (the shot set's
     // size bound, 8 (N + 64 nrows ncols) C <= 2^32, keeps a level outside the column
     // outside the field)
-    const unsigned oI = 2u * l.oI, oHX = 2u * l.oHX, oHY = 2u * l.oHY;
+    const unsigned oI = 2u * l.oI;
     const int gi = t.i0 + l.li, gj = t.j0 + l.lj;
     const bool col_interior =
         l.valid && gi >= 1 && gi <= a.nrows - 2 && gj >= 1 && gj <= a.ncols - 2;
@@ -581,28 +645,19 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         *any_change = 0;
     }
 
-    // halo loads only in the lanes that use them (vector-memory instructions bound
-    // the sweep; a wave without a halo lane skips them)
-    const bool hxuse = l.uxm || l.uxp, hyuse = l.uym || l.uyp;
+    // the halo faces come through the block's LDS stage (HaloStage)
+    using HS = HaloStage<TI, TJ, C, L>;
+    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, l, s1, s2, s3, 2u);
+    float2 hl[HS::K];  // this thread's stage values of the next batch
     auto ld2 = [&](unsigned o) { return fload2<FLOW>(RLT, o); };
-    auto load_batch = [&](int pb, float2 *ow, float2 *hx, float2 *hy) {
+    auto load_batch = [&](int pb, float2 *ow) {
         const unsigned dm = (unsigned)(pb - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++) ow[u] = ld2(oI + dm + (unsigned)(u + 2) * st);
-        if (hxuse) {
+        const unsigned dp = (unsigned)pb * st;
 #pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = ld2(oHX + dm + (unsigned)u * st);
-        } else {
-#pragma unroll
-            for (int u = 0; u < L; u++) hx[u] = make_float2(0.0f, 0.0f);
-        }
-        if (hyuse) {
-#pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = ld2(oHY + dm + (unsigned)u * st);
-        } else {
-#pragma unroll
-            for (int u = 0; u < L; u++) hy[u] = make_float2(0.0f, 0.0f);
-        }
+        for (int k = 0; k < HS::K; k++)
+            hl[k] = hs.use[k] ? ld2(hs.addr[k] + dp) : make_float2(0.0f, 0.0f);
     };
     if constexpr (FLOW) {
         flow_wait(*fw, L - 1);
@@ -612,8 +667,8 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     float2 vprev = make_float2(0.0f, 0.0f);
     float2 vcur = l.valid && m0 == 0 ? ld2(oI) : make_float2(0.0f, 0.0f);
     float2 vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
-    float2 qa[3][L], qb[3][L];
-    load_batch(0, qa[0], qa[1], qa[2]);
+    float2 qa[1][L], qb[1][L], hx[L], hy[L];
+    load_batch(0, qa[0]);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();
 
@@ -625,12 +680,16 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         sval[u] = 0.0f;
         chg[u] = false;
     }
-    auto batch = [&](int pb, float2 (*q)[L], float2 (*nq)[L]) {
+    auto batch = [&](int pb, int par, float2 (*q)[L], float2 (*nq)[L]) {
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
             flow_prefetch(*fw);
         }
+        float2 *Hs = hstage + par * HS::NS;
+#pragma unroll
+        for (int k = 0; k < HS::K; k++)
+            if (HS::NS % HS::NT == 0 || tid + k * HS::NT < HS::NS) Hs[tid + k * HS::NT] = hl[k];
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
@@ -642,7 +701,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             for (int u = 0; u < L; u++)
                 if (chg[u]) fstore<FLOW>(RLT, oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pbo + L, nq[0], nq[1], nq[2]);
+        load_batch(pbo + L, nq[0]);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
@@ -653,13 +712,20 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             if constexpr (FLOW) {
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
             }
+            if (u == 0) {  // the batch's halo pairs of this lane's faces
+#pragma unroll
+                for (int v = 0; v < L; v++) {
+                    hx[v] = Hs[hs.xs + v * HS::HSU];
+                    hy[v] = Hs[hs.ys + v * HS::HSU];
+                }
+            }
             // branch-free: the (lambda, T) pairs of the neighbours come in one LDS read
             // each, the update is computed in every lane and selected afterwards
             const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            const float2 xm = l.uxm ? q[1][u] : pxm;
-            const float2 xp = l.uxp ? q[1][u] : pxp;
-            const float2 ym = l.uym ? q[2][u] : pym;
-            const float2 yp = l.uyp ? q[2][u] : pyp;
+            const float2 xm = l.uxm ? hx[u] : pxm;
+            const float2 xp = l.uxp ? hx[u] : pxp;
+            const float2 ym = l.uym ? hy[u] : pym;
+            const float2 yp = l.uyp ? hy[u] : pyp;
             // depth k-1 / k+1 by absolute depth (levels m-1 / m+1 are depths
             // k-s3 / k+s3): every ordering sums the terms in one order, so a
             // stationary node stays bitwise stationary
@@ -699,9 +765,9 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
     for (int pb = 0;; pb += 2 * L) {
-        batch(pb, qa, qb);
+        batch(pb, 0, qa, qb);
         if (pb + L >= pend) break;
-        batch(pb + L, qb, qa);
+        batch(pb + L, 1, qb, qa);
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -724,13 +790,15 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
 {
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float2 pub[2 * PB];
+    __shared__ float2 hstage[2 * HaloStage<TI, TJ, C, L>::NS];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr, nullptr);
+    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, hstage, &any_change, nullptr,
+                                              nullptr);
 }
 
 // ======================================================= dataflow kernels
@@ -893,6 +961,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_phase(f);
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float pub[2 * PB];
+    __shared__ float hstage[2 * HaloStage<TI, TJ, C, L>::NS];
     __shared__ int any_change;
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
@@ -906,7 +975,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     fwd_tile<TI, TJ, C, L, UNI, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), S3POS ? 1 : -1,
-                                      six, pub, &any_change, &fw, self);
+                                      six, pub, hstage, &any_change, &fw, self);
     flow_finish(f, self);
 }
 
@@ -916,6 +985,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_phase(f);
     constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
     __shared__ float2 pub[2 * PB];
+    __shared__ float2 hstage[2 * HaloStage<TI, TJ, C, L>::NS];
     __shared__ int any_change;
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
@@ -929,7 +999,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
-                                        &any_change, &fw, self);
+                                             hstage, &any_change, &fw, self);
     flow_finish(f, self);
 }
 
