@@ -20,6 +20,7 @@ the CPU, one MPI rank) and keeps, in driver2d.npz / driver3d.npz:
                  sensitivity, which bounds how closely any other arithmetic can
                  follow its trajectory
   vp1, vpF    -- the model after iteration 1 and the final model
+  vp1_ens, vpF_ens -- the same two models of each perturbed run
 Observed picks come from the FP64 oracle on a seeded "true" model plus 2 ms noise.
 """
 import os
@@ -151,10 +152,11 @@ def run(exe, d, args, timeout):
     return r.returncode, r.stdout
 
 
-def ensemble(c, write, exe, args, errfile, col=None):
-    """Misfit and step sequences of the reference from perturbed starting models."""
+def ensemble(c, write, exe, args, errfile, models, shape, col=None):
+    """Misfit and step sequences, and the models after iteration 1 and at the end,
+    of the reference from perturbed starting models."""
     rng = np.random.default_rng(7)
-    Es, Ss = [], []
+    Es, Ss, M1, MF = [], [], [], []
     for _ in range(NENS):
         cp = dict(c)
         cp["v0"] = c["v0"] * (1 + PERTURB * rng.standard_normal(c["v0"].shape))
@@ -165,9 +167,12 @@ def ensemble(c, write, exe, args, errfile, col=None):
             E = np.loadtxt(os.path.join(d, errfile), ndmin=2 if col is not None else 1)
             Es.append(E[:, col] if col is not None else E)
             Ss.append(np.array(steps_from_log(log)))
+            M1.append(F.read_bin(os.path.join(d, models[0]), shape))
+            MF.append(F.read_bin(os.path.join(d, models[1]), shape))
     n = max(len(e) for e in Es)
     pad = lambda v: np.concatenate([v, np.full(n - len(v), np.nan)])
-    return np.stack([pad(e) for e in Es]), np.stack([pad(v) for v in Ss])
+    return (np.stack([pad(e) for e in Es]), np.stack([pad(v) for v in Ss]), np.stack(M1),
+            np.stack(MF))
 
 
 def main():
@@ -184,8 +189,10 @@ def main():
                     trials=trials_from_log(log),
                     vp1=F.read_bin(os.path.join(d, "m_1"), (c["nr"], c["nc"])),
                     vpF=F.read_bin(os.path.join(d, "final.vp"), (c["nr"], c["nc"])))
-        out2["E_ens"], out2["steps_ens"] = ensemble(c, write2d, os.path.join(REFBIN, "RT_PFAST2D"),
-                                                    ["50", "3"], "m_GlobalErr_TR_T_R.txt", col=0)
+        (out2["E_ens"], out2["steps_ens"], out2["vp1_ens"],
+         out2["vpF_ens"]) = ensemble(c, write2d, os.path.join(REFBIN, "RT_PFAST2D"), ["50", "3"],
+                                     "m_GlobalErr_TR_T_R.txt", ("m_1", "final.vp"),
+                                     (c["nr"], c["nc"]), col=0)
         print(f"2D: {len(E)} iterations, E {E[0, 0]:.6g} -> {E[-1, 0]:.6g}, steps "
               f"{out2['steps'][:4]}")
         np.savez_compressed(os.path.join(HERE, "driver2d.npz"), **out2)
@@ -201,8 +208,9 @@ def main():
                     E=E, steps=np.array(steps_from_log(log)), trials=trials_from_log(log),
                     vp1=F.read_bin(os.path.join(d, "m.vp1"), shape),
                     vpF=F.read_bin(os.path.join(d, "final.vp"), shape))
-        out3["E_ens"], out3["steps_ens"] = ensemble(c, write3d, os.path.join(REFBIN, "T_PFAST3D"),
-                                                    ["50"], "m_GlobalErr.txt")
+        (out3["E_ens"], out3["steps_ens"], out3["vp1_ens"],
+         out3["vpF_ens"]) = ensemble(c, write3d, os.path.join(REFBIN, "T_PFAST3D"), ["50"],
+                                     "m_GlobalErr.txt", ("m.vp1", "final.vp"), shape)
         # resume from the model after iteration 2 (the reference's vp0file-digit convention):
         # its inputs from this run, and the reference's resumed trajectory
         res = dict(vp2=F.read_bin(os.path.join(d, "m.vp2"), shape),

@@ -17,6 +17,8 @@ input moves its own step 1 by up to 80% in 3D and its final misfit by up to 20% 
   * per iteration, misfits and accepted steps within max(1%, 3x the reference's
     own spread under the perturbation), while every perturbed run is still going;
   * the inversion reaches the reference's final misfit: at most 1% above it;
+  * the models after iteration 1 and at the end differ from the reference's by at
+    most 3x the reference's own spread (its perturbed runs against its run);
   * conditional parity along our own path: the FP64 oracle's misfit at every model
     the driver writes equals the misfit the driver logged for it (1e-4; in 2D the
     transmission and reflection parts each), and in 3D, at the driver's own
@@ -90,6 +92,19 @@ def _compare(E, steps, log, g):
     assert E[-1] <= (1 + E_TOL) * Eref[-1]
 
 
+def _model_error(label, ours, ref, v0, ens):
+    """The driver's model against the reference's FP64 one (relative L2 of the
+    difference over the reference model, and over the reference's own update
+    ref - v0), bounded by the reference's own spread: 3x the largest difference
+    between its runs from the perturbed starting models (ens) and ref."""
+    d = np.linalg.norm(ours - ref)
+    r_model, r_update = d / np.linalg.norm(ref), d / np.linalg.norm(ref - v0)
+    spread = max(np.linalg.norm(e - ref) for e in ens) / np.linalg.norm(ref)
+    print(f"{label}: |ours - ref| / |ref| = {r_model:.3e} (/ |ref - v0| = {r_update:.3e}); "
+          f"reference's own spread {spread:.3e}", flush=True)
+    assert r_model <= max(3 * spread, 1e-6)
+
+
 def _steps(log):
     import re
     return np.array([float(v) for v in re.findall(r"Stepsize=([-0-9.eE+]+)", log)])
@@ -112,6 +127,10 @@ def test_rt2d_driver_matches_reference_trajectory(tmp_path):
     assert _rel(E[0, 1], g["Et"][0]) <= 1e-4 and _rel(E[0, 2], g["Er"][0]) <= 1e-4
     _compare(E[:, 0], _steps(log), log, g)
     _conditional_parity2d(d, E)
+    _model_error("model after iteration 1", F.read_bin(os.path.join(d, "m_1"), (nr, nc)), g["vp1"],
+                 g["v0"], g["vp1_ens"])
+    _model_error("final model", F.read_bin(os.path.join(d, "final.vp"), (nr, nc)), g["vpF"],
+                 g["v0"], g["vpF_ens"])
 
 
 def _conditional_parity2d(d, E):
@@ -152,6 +171,11 @@ def test_t3d_driver_matches_reference_trajectory(tmp_path):
     E = np.loadtxt(os.path.join(d, "m_GlobalErr.txt"), ndmin=1)
     _compare(E, _steps(log), log, g)
     _conditional_parity3d(d, g, E, log)
+    shape = (nz, nr, nc)
+    _model_error("model after iteration 1", F.read_bin(os.path.join(d, "m.vp1"), shape), g["vp1"],
+                 g["v0"], g["vp1_ens"])
+    _model_error("final model", F.read_bin(os.path.join(d, "final.vp"), shape), g["vpF"],
+                 g["v0"], g["vpF_ens"])
 
 
 def _conditional_parity3d(d, g, E, log):
