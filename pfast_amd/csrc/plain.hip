@@ -181,7 +181,8 @@ struct Fast32 {
 // two to LDS (double-buffered by step parity, one barrier per step).  Memory traffic
 // is batched over W2L steps: one vmcnt(0) wait, the stores of the previous batch's
 // changed nodes, then the loads of the next batch (own row two columns ahead and the
-// node inputs), so the steps themselves touch registers and LDS only.  The loads are
+// node inputs: the model's FP32 slowness, (float)(1.0 / vp) as the plane loop forms it,
+// and the reflector depth), so the steps themselves touch registers and LDS only.  The loads are
 // raw buffer loads issued for every column of the batch: one outside the row or the
 // grid reads 0 or a neighbouring row's value, never faults, and is never used (edge
 // columns and rows take the reference's one-sided rules, and only in-range nodes are
@@ -206,17 +207,17 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
     const float fdx = (float)g.dx, fdz = (float)g.dz, fwx = (float)g.wx, fwz = (float)g.wz;
     const double zlim = (i + 1) * g.dz;
     const unsigned N = (unsigned)nrows * ncols;
-    const rsrc_t RU = make_rsrc(U, 4u * N), RV = make_rsrc(vp, 8u * N);
+    const rsrc_t RU = make_rsrc(U, 4u * N), RS = make_rsrc(g.slow, 4u * N);
     const rsrc_t RZ = make_rsrc(Zr, mode == 1 ? 24u * ncols : 0u);
     auto col = [&](int w) { return s2 > 0 ? w : ncols - 1 - w; };
     auto in = [&](int w) { return valid && w >= 0 && w < ncols; };
     auto ld = [&](int w) { return bload(RU, 4u * (unsigned)(rb + col(w))); };
-    // raw inputs of the batch's nodes: own row two columns ahead, velocity, and the
+    // raw inputs of the batch's nodes: own row two columns ahead, slowness, and the
     // reflector depth of the column (mode 1).  The reflection rows of mode 2 (k2vec)
     // may live in LDS and are read at the batch start through their own pointer.
     struct Q {
-        float u[L];
-        double v[L], z[L];
+        float u[L], s[L];
+        double z[L];
     };
     auto load_batch = [&](int pb, Q &q) {
 #pragma unroll
@@ -224,7 +225,7 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
             const int w = pb + u - t;
             const int j = col(w);
             q.u[u] = ld(w + 2);
-            q.v[u] = bload_d(RV, 8u * (unsigned)(rb + j));
+            q.s[u] = bload(RS, 4u * (unsigned)(rb + j));
             if (mode == 1) q.z[u] = bload_d(RZ, 24u * (unsigned)j);
         }
     };
@@ -248,14 +249,12 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
     };
     auto batch = [&](int pb, const Q &q, Q &nq) {
         wait_vm_all();
-        // node flags (src/RT_pfsm2d.h:174, :309) and slowness from the arrived inputs,
-        // before this batch's stores and loads are in flight
-        float sl[L];
+        // node flags (src/RT_pfsm2d.h:174, :309) from the arrived inputs, before this
+        // batch's stores and loads are in flight
         bool upd[L];
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int w = pb + u - t, j = col(w);
-            sl[u] = (float)(1.0 / q.v[u]);
             const bool notsrc = !(i == si && j == sj);
             bool up;
             if (mode == 2) {
@@ -264,10 +263,10 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
             } else {
                 up = in(w) && (mode == 0 ? notsrc : notsrc && zlim <= q.z[u]);
             }
-            // materialised here: left to the compiler, the FP64 division and compare
-            // sink into each step's update and lengthen the step's dependent chain
+            // materialised here: left to the compiler, the FP64 compare sinks into each
+            // step's update and lengthens the step's dependent chain
             int upi = up;
-            asm volatile("" : "+v"(sl[u]), "+v"(upi));
+            asm volatile("" : "+v"(upi));
             upd[u] = upi != 0;
         }
         store_batch(pb - L);
@@ -290,7 +289,7 @@ __device__ bool fsm2d_window(const double *__restrict__ vp, float *__restrict__ 
                 const bool right = s2 > 0 ? w == ncols - 1 : w == 0;
                 const float b = top ? up : bottom ? um : (um < up ? um : up);
                 const float a = left ? lp : right ? lm : (lp < lm ? lp : lm);
-                const float qv = eik2d_f32(a, b, sl[u], fdx, fdz, fwx, fwz);
+                const float qv = eik2d_f32(a, b, q.s[u], fdx, fdz, fwx, fwz);
                 nw = cur < qv ? cur : qv;
                 changed |= nw != cur;
             }
@@ -443,7 +442,7 @@ __device__ int wg_fsm2d(const double *__restrict__ vp, typename P::real *__restr
             const int s1 = (o < 2) ? -1 : 1, s2 = (o & 1) ? 1 : -1;
             const bool first = o == 0, last = o == 3;
             if constexpr (!P::exact) {
-                if (g.window && blockDim.x == W2NT && nrows <= W2NT) {
+                if (g.window && g.slow && blockDim.x == W2NT && nrows <= W2NT) {
                     if (fsm2d_window(vp, U, si, sj, mode, Zr, k2vec, g, s1, s2)) *s_flag = 1;
                     __syncthreads();
                     continue;

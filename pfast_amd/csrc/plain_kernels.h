@@ -8,6 +8,7 @@ struct Grid2 {
     // FP32 batched form: 1 = row-window sweeps and independent solves of a source in
     // blocks of their own (default), 0 = plane loop, one block per source (same results)
     int window = 1;
+    const float *slow = nullptr;  // (float)(1 / vp) per node (the model's), for the row window
 };
 
 namespace pfast {

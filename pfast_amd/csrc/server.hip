@@ -827,7 +827,8 @@ int pfast_server_add_model2d(pfast_server *S, const double *vp, int nrows, int n
 {
     ARGCHK(S && vp && model_id, "NULL argument");
     HIPCHK(hipSetDevice(S->device));
-    const size_t bytes = round2m((size_t)nrows * ncols * 8) + round2m((size_t)NumofR * ncols * 24 + 8);
+    const size_t bytes = round2m((size_t)nrows * ncols * 8) + round2m((size_t)NumofR * ncols * 24 + 8) +
+                         round2m((size_t)nrows * ncols * 4);
     ARGCHK(S->in_use + bytes <= S->budget, "resident models exceed the server's memory budget");
     SModel M;
     M.dim = 2;

@@ -12,6 +12,7 @@ struct pfast_model2d {
     int nrows, ncols, NumofR;
     double dx, dz;
     double *vp = nullptr;  // N doubles, reference layout vp[i * ncols + j]
+    float *slow = nullptr; // N floats, (float)(1 / vp): the FP32 sweeps' slowness
     double *Z = nullptr;   // NumofR * ncols * 3 (depth, n_z, n_x), reference layout
     int nshots = 0;
 };

@@ -49,6 +49,14 @@ __global__ void grad2d_reduce(const float *__restrict__ LT, const float *__restr
     gRT[n] = a * (1. - w) + b * w;
 }
 
+// The FP32 sweeps' slowness, 1 / vp rounded to FP32 once per model (the value the
+// plane loop forms per node as (float)(1.0 / vp[n])).
+__global__ void slow2d(const double *__restrict__ vp, float *__restrict__ slow, long N)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) slow[n] = (float)(1.0 / vp[n]);
+}
+
 __global__ void gather2d(const float *__restrict__ F, long N, double *__restrict__ out)
 {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,6 +84,7 @@ Grid2 grid_of(const pfast_model2d *m)
     g.wx = 1.0 / (m->dx * m->dx);
     g.wz = 1.0 / (m->dz * m->dz);
     g.window = m->ctx->sweep2d == 0;
+    g.slow = m->slow;
     return g;
 }
 
@@ -189,6 +198,10 @@ int pfast_model2d_create(pfast_ctx *ctx, const double *vp, int nrows, int ncols,
     const long N = (long)nrows * ncols;
     int rc = pfast::dupload(ctx, &m->vp, vp, N);
     if (!rc) rc = pfast::dupload(ctx, &m->Z, Z, (size_t)NumofR * ncols * 3);
+    if (!rc) rc = pfast::dalloc(ctx, &m->slow, N);
+    if (!rc)
+        hipLaunchKernelGGL(slow2d, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream,
+                           m->vp, m->slow, N);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) {
         pfast::set_error("upload of the 2D model failed");
         rc = PFAST_ERR_HIP;
@@ -206,6 +219,9 @@ int pfast_model2d_set_vp(pfast_model2d *m, const double *vp)
     ARGCHK(m && vp, "NULL argument");
     const long N = (long)m->nrows * m->ncols;
     HIPCHK(hipMemcpyAsync(m->vp, vp, N * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+    hipLaunchKernelGGL(slow2d, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, m->ctx->stream,
+                       m->vp, m->slow, N);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(m->ctx->stream));
     return PFAST_OK;
 }
@@ -218,6 +234,7 @@ int pfast_model2d_destroy(pfast_model2d *m)
     m->ctx->nmodels--;
     pfast::dfree(m->ctx, m->vp);
     pfast::dfree(m->ctx, m->Z);
+    pfast::dfree(m->ctx, m->slow);
     delete m;
     return PFAST_OK;
 }
