@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--nsrc", type=int, default=400)
     ap.add_argument("--nrec", type=int, default=2000)
     ap.add_argument("--maxswp", type=int, default=100)
-    ap.add_argument("--plain-sources", type=int, default=2,
+    ap.add_argument("--plain-sources", type=int, default=4,
                     help="sources run through the single-source FP64 GPU form")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="CPU baseline threads (default: the 16-core CPU share of one GPU, "
@@ -300,7 +300,7 @@ def main():
                     help="keep --sweep, --schedule and --check-interval as given")
     ap.add_argument("--check-interval", type=int, default=1,
                     help="sweep cycles between two convergence checks on the host")
-    ap.add_argument("--graphs", choices=["on", "off"], default="off",
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="replay the cycles between checks from captured hipGraphs (the "
                          "bench line also carries the step time with the other setting)")
     ap.add_argument("--ab-steps", type=int, default=2,
