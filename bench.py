@@ -176,6 +176,14 @@ def tomo2d(args):
             sh.run(adjoint=True, MAXSwp=500)
             sh.gradient(0.5)
             best = min(best, time.perf_counter() - t0)
+        # roofline: algorithmic bytes of the node-sweeps the reference performs (forward:
+        # every node per ordering, T read and written and the slowness read; adjoints:
+        # interior nodes, lambda read and written and T read: 12 B each) over the time
+        cyc = np.asarray(sh.info()[0], np.float64)
+        nr_, nc_ = cfg["nrows"], cfg["ncols"]
+        fwd_sweeps = cyc[:, :1 + 2 * R].sum() * 4 * nr_ * nc_
+        adj_sweeps = cyc[:, 1 + 2 * R:].sum() * 4 * (nr_ - 2) * (nc_ - 2)
+        gbps = 12.0 * (fwd_sweeps + adj_sweeps) / best / 1e9
         sh.close()
         m0.close()
         t0 = time.perf_counter()
@@ -189,7 +197,12 @@ def tomo2d(args):
         res[name] = {"gradient_ms": round(best * 1e3, 2),
                      "source_solves_per_s": round(nsrc / best, 1),
                      "plain_fp64_one_source_ms": round(plain * 1e3, 2),
-                     "ratio_vs_plain_fp64": round(nsrc / best * plain, 1)}
+                     "ratio_vs_plain_fp64": round(nsrc / best * plain, 1),
+                     "roofline": {"bound": "hbm", "limited_by": "latency: one dependent "
+                                  "chain of row and column steps per sweep and source",
+                                  "achieved": round(gbps, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 4),
+                                  "node_sweeps": fwd_sweeps + adj_sweeps}}
         log(f"2D {name}: {best * 1e3:.1f} ms per gradient ({nsrc / best:.0f} source-solves/s), "
             f"plain FP64 {plain * 1e3:.1f} ms per source")
     ctx.close()

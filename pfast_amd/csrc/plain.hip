@@ -154,17 +154,22 @@ struct Fast32 {
     {
         return eik2d_f32(a, b, s, (float)g.dx, (float)g.dz, (float)g.wx, (float)g.wz);
     }
+    // the batched 3D adjoint's form (adj_acc): each difference formed once, fma
+    // accumulation from +0, any stencil traveltime at INF by one compare of the maximum,
+    // computed in every lane and selected without a branch
     __device__ static real adj(real tc, real txp, real txm, real tzp, real tzm, real lxp,
                                real lxm, real lzp, real lzm, const Grid2 &g)
     {
-        const float I = PFAST_INF_F;
-        if (tc >= I || txp >= I || txm >= I || tzp >= I || tzm >= I) return 0.0f;
-        float den = 0.0f;
-        float num = adj_term(tc, txp, lxp, (float)g.wx, den);
-        num += adj_term(tc, txm, lxm, (float)g.wx, den);
-        num += adj_term(tc, tzp, lzp, (float)g.wz, den);
-        num += adj_term(tc, tzm, lzm, (float)g.wz, den);
-        return fabsf(den) > 1e-9f ? num * frcp(den) : 0.0f;
+        const float wx = (float)g.wx, wz = (float)g.wz;
+        const float tmax = fmaxf(fmaxf(fmaxf(tc, txp), fmaxf(txm, tzp)), tzm);
+        float num = 0.0f, den = 0.0f;
+        adj_acc<false>(txp - tc, lxp, wx, num, den);
+        adj_acc<false>(txm - tc, lxm, wx, num, den);
+        adj_acc<false>(tzp - tc, lzp, wz, num, den);
+        adj_acc<false>(tzm - tc, lzm, wz, num, den);
+        float r = num * frcp(den);
+        asm volatile("" : "+v"(r));
+        return tmax < PFAST_INF_F && den > 1e-9f ? r : 0.0f;
     }
     __device__ static real slow(const double *vp, long n) { return (float)(1.0 / vp[n]); }
 };

@@ -169,25 +169,16 @@ __device__ __forceinline__ float eik2d_f32(float a, float b, float s, float dx, 
     return q1 <= hi ? q1 : q2;
 }
 
-// FP32 adjoint update from differences (same upwind coefficients as
-// src/T_padjoint3d.h:51-68: a1-a2 = (max(tc-txp,0)+max(tc-txm,0))/dx, ...).
-// Returns lambda; T values are non-negative magnitudes.
-// The neighbour's lambda enters only when it is upwind (tn > tc); the select
-// keeps the result independent of downwind lambdas, including the sign of a zero
-// (0 * negative = -0 would otherwise couple a node to its downwind neighbours and
-// let the sign bit oscillate between sweeps).
-__device__ __forceinline__ float adj_term(float tc, float tn, float ln, float w, float &den)
-{
-    den = fmaf(w, fmaxf(tc - tn, 0.0f), den);
-    return tn > tc ? w * (tn - tc) * ln : 0.0f;
-}
-
-// Batched 3D adjoint accumulation from the difference d = tn - tc of one neighbour:
-// den += w max(-d, 0), num += w max(d, 0) ln, the product added by one fma.  num and
-// den start at +0: a downwind neighbour's product is a zero whose sign is that of ln,
-// and a signed zero added to +0 (or to a non-zero sum) leaves it unchanged, so the
-// result never depends on downwind lambdas, not even through the sign of a zero.
-// Uniform spacing (UNI): every w is the same and cancels in num / den; it is left out.
+// FP32 adjoint accumulation (same upwind coefficients as src/T_padjoint3d.h:51-68 and
+// src/RT_padjoint2d.h:42-65: den = sum over neighbours of max(tc - tn, 0) / h^2, num =
+// sum of max(tn - tc, 0) lambda_n / h^2) from the difference d = tn - tc of one
+// neighbour: den += w max(-d, 0), num += w max(d, 0) ln, the product added by one fma.
+// T values are non-negative magnitudes.  num and den start at +0: a downwind
+// neighbour's product is a zero whose sign is that of ln, and a signed zero added to +0
+// (or to a non-zero sum) leaves it unchanged, so the result never depends on downwind
+// lambdas, not even through the sign of a zero (a -0 there would couple a node to its
+// downwind neighbours and let its sign bit oscillate between sweeps).  Uniform spacing
+// (UNI): every w is the same and cancels in num / den; it is left out.
 template <bool UNI>
 __device__ __forceinline__ void adj_acc(float d, float ln, float w, float &num, float &den)
 {

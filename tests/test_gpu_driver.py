@@ -16,7 +16,10 @@ input moves its own step 1 by up to 80% in 3D and its final misfit by up to 20% 
     within 1e-2 (relative);
   * per iteration, misfits and accepted steps within max(1%, 3x the reference's
     own spread under the perturbation), while every perturbed run is still going;
-  * the inversion reaches the reference's final misfit: at most 1% above it;
+  * the inversion ends no worse than the reference's runs typically end: its final
+    misfit at most 1% above the median of the reference's final misfit and the last
+    misfits of its perturbed runs (in 2D these end 5% to 18% above the reference's,
+    in 3D two end within 3% of it and two 9x and 29x above it);
   * the models after iteration 1 and at the end differ from the reference's by at
     most 3x the reference's own spread (its perturbed runs against its run);
   * conditional parity along our own path: the FP64 oracle's misfit at every model
@@ -89,7 +92,9 @@ def _compare(E, steps, log, g):
             stol = max(E_TOL, 3 * sspread)
             print(f"  step {steps[k]:.6g} ref {sref[k]:.6g} (tol {stol:.3g})", flush=True)
             assert _rel(steps[k], sref[k]) <= stol
-    assert E[-1] <= (1 + E_TOL) * Eref[-1]
+    last = [row[~np.isnan(row)][-1] for row in Eens]
+    print(f"final: E {E[-1]:.6g} ref {Eref[-1]:.6g} perturbed runs {last}", flush=True)
+    assert E[-1] <= (1 + E_TOL) * np.median([Eref[-1], *last])
 
 
 def _model_error(label, ours, ref, v0, ens):
