@@ -98,8 +98,8 @@ def cpu_baseline(cfg, nthreads, nsamp, maxswp):
 def measured_traffic(kernel, sweep_name, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (profiles/traffic.json, copy-calibrated), when they were taken
-    on this same workload, sweep variant and tile-skip setting (a launch moves the same
-    bytes whether or not other launches overlap it); else None."""
+    on this same workload, sweep variant, tile-skip and pairing setting (a launch
+    moves the same bytes whether or not other launches overlap it); else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
@@ -108,7 +108,8 @@ def measured_traffic(kernel, sweep_name, args):
         return None
     if (t.get("sweep") != sweep_name or t.get("nsrc") != args.nsrc or t.get("nrec") != args.nrec
             or t.get("grid") != [101, 201, 201]
-            or t.get("tile_skip", False) != (not args.no_tile_skip)):
+            or t.get("tile_skip", False) != (not args.no_tile_skip)
+            or t.get("paired", False) != (args.paired == "on")):
         return None
     return t.get("bytes_per_launch", {}).get(kernel)
 
@@ -322,6 +323,9 @@ def main():
                     help="batched 3D sweep schedule: dataflow launches (two per cycle), one "
                          "launch per tile anti-diagonal, or per cycle the one that suits its "
                          "active chunks (same results)")
+    ap.add_argument("--paired", choices=["on", "off"], default="on",
+                    help="per-diagonal launches pair orderings 2p and 2p+1, the second "
+                         "trailing by ntj + 1 tile diagonals (same results)")
     ap.add_argument("--no-tile-skip", action="store_true",
                     help="sweep every tile (exact tile skipping off; same results)")
     ap.add_argument("--streams", type=int, default=3,
@@ -373,6 +377,7 @@ def main():
     ctx = pfast_amd.Context(0)
     ctx.set_streams(args.streams)
     ctx.set_tile_skip(not args.no_tile_skip)
+    ctx.set_paired(args.paired == "on")
     ctx.set_dataflow({"auto": 2, "dataflow": 1, "diagonal": 0}[args.schedule])
     ctx.set_sweep(args.sweep)
     ctx.set_check_interval(args.check_interval)
@@ -511,7 +516,7 @@ def main():
                    "parallelism": "all sources batched on 1 GPU", "sweep": sweep_name,
                    "schedule": args.schedule, "check_interval": args.check_interval,
                    "graphs": args.graphs, "autotuned": bool(args.autotune),
-                   "streams": args.streams,
+                   "streams": args.streams, "paired": args.paired,
                    "tile_skip": not args.no_tile_skip},
         "roofline": roofline,
         "graphs": graphs_ab,

@@ -224,6 +224,13 @@ int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams);
  * sweep in an ordering changed nothing, and whose nodes and neighbour tiles have
  * not changed since, is not swept again in that ordering; results are the same. */
 int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable);
+/* Paired per-diagonal launches of the batched 3D sweeps (on by default): orderings 2p
+ * and 2p+1 of a cycle, which differ only in the column direction (the innermost loop of
+ * src/T_pfsm3d.h:158-160 and src/T_padjoint3d.h:139-141), share launches, the second
+ * trailing the first by ntj + 1 tile anti-diagonals (ntj: tiles across the columns), so
+ * that the thin tail diagonals of one ordering run beside the head of the next.  Every
+ * tile still reads exactly the values of the sequential sweep; results are the same. */
+int pfast_ctx_set_paired(pfast_ctx *ctx, int enable);
 /* Schedule of the batched 3D sweep cycles (the loops of src/T_pfsm3d.h:152-179 and
  * src/T_padjoint3d.h:135-163, same node updates in the same order under every mode):
  * 0 = one launch per tile anti-diagonal and ordering; 1 = dataflow launches, two per

@@ -175,6 +175,11 @@ class Context:
         """Exact tile skipping of the batched 3D sweeps (default on; same results)."""
         check(lib().pfast_ctx_set_tile_skip(self.h, 1 if enable else 0))
 
+    def set_paired(self, enable=True):
+        """Pair orderings 2p and 2p+1 of a batched 3D cycle in shared per-diagonal
+        launches, the second trailing by ntj + 1 diagonals (default on; same results)."""
+        check(lib().pfast_ctx_set_paired(self.h, 1 if enable else 0))
+
     def set_dataflow(self, mode=True):
         """Schedule of the batched 3D sweep cycles: False/0 one launch per tile
         anti-diagonal, True/1 dataflow launches (a tile starts as soon as the values it

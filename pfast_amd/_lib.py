@@ -62,6 +62,7 @@ _SIGS = {
     "pfast_ctx_set_sweep2d": [vpp, i],
     "pfast_ctx_set_streams": [vpp, i],
     "pfast_ctx_set_tile_skip": [vpp, i],
+    "pfast_ctx_set_paired": [vpp, i],
     "pfast_ctx_set_dataflow": [vpp, i],
     "pfast_ctx_set_check_interval": [vpp, i],
     "pfast_ctx_set_graphs": [vpp, i],

@@ -38,6 +38,10 @@ struct SweepArgs {
     // change nothing: skipped.
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
+    // A paired launch (plan.h) sweeps two orderings that differ in the column direction
+    // only: blocks of tile index >= ntA take diagonal DB of ordering oB, the column
+    // direction flipped, at epoch epochB.  ntA = INT_MAX: one ordering.
+    int ntA, DB, oB, epochB;
     // Phase state on the device (NULL: the host passes every cycle's values): the
     // cycle number phase[0] (1-based) that the cycle's first kernel advanced; epoch then
     // counts from the cycle's start, to which the kernel adds (phase[0] - 1) * ep_cycle.

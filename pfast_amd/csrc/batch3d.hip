@@ -33,11 +33,18 @@ struct TileCtx {
 };
 
 template <int TI, int TJ>
-__device__ __forceinline__ TileCtx decode_tile(const SweepArgs &a, int D, int s1, int s2)
+__device__ __forceinline__ TileCtx decode_tile(SweepArgs &a, int D, int s1, int &s2)
 {
     TileCtx t;
     const int slot = blockIdx.x % a.nact_p8;
-    const int tt = blockIdx.x / a.nact_p8;
+    int tt = blockIdx.x / a.nact_p8;
+    if (tt >= a.ntA) {  // the second ordering of a paired launch
+        tt -= a.ntA;
+        D = a.DB;
+        s2 = -s2;
+        a.epoch = a.epochB;
+        a.o = a.oB;
+    }
     t.live = slot < a.nact;
     t.chunk = t.live ? a.active[slot] : 0;
     const int tlo = D - (a.ntj - 1) > 0 ? D - (a.ntj - 1) : 0;

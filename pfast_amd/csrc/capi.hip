@@ -18,6 +18,7 @@
 #include "context.h"
 #include "pfast_internal.h"
 #include "plain_kernels.h"
+#include "plan.h"
 #include "shots3d.h"
 
 namespace pfast {
@@ -227,6 +228,13 @@ int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable)
 {
     ARGCHK(ctx, "ctx is NULL");
     ctx->tile_skip = enable ? 1 : 0;
+    return PFAST_OK;
+}
+
+int pfast_ctx_set_paired(pfast_ctx *ctx, int enable)
+{
+    ARGCHK(ctx, "ctx is NULL");
+    ctx->paired = enable ? 1 : 0;
     return PFAST_OK;
 }
 
@@ -755,8 +763,8 @@ int issue_batch3d(pfast_shots3d *s, bool adjoint, bool flow, int ncycles, int nl
                 hipStream_t sk = k ? ctx->aux[k - 1] : st;
                 const int *lk = s->active + off[k];
                 const int nk = off[k + 1] - off[k];
-                int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, 0, 8, sk, sp, s->phase)
-                                 : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, 0, 8, sk, sp, s->phase);
+                int rc = adjoint ? pfast::issue_cycle3d(ctx, g, s->chunks, nullptr, 0, lk, nk, s->changed, sk, sp, s->phase)
+                                 : pfast::issue_cycle3d(ctx, g, s->chunks, lk, nk, nullptr, 0, s->changed, sk, sp, s->phase);
                 if (rc) return rc;
             }
             for (int k = 1; k < K; k++) {
@@ -962,6 +970,7 @@ SweepArgs sweep_args(const pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunk
     const bool skipping = skip && sv.kind == 0;
     a.tmod = skipping ? skip->tmod : nullptr;
     a.tclean = skipping ? skip->tclean : nullptr;
+    a.ntA = 0x7fffffff;
     return a;
 }
 
@@ -1023,16 +1032,24 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
     return PFAST_OK;
 }
 
-int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
-                      hipStream_t st, const TileSkip *skip, const int *dphase)
+int group_orderings3d(const pfast_ctx *ctx) { return plan_group_orderings(ctx->paired != 0); }
+
+int group_steps3d(const pfast_ctx *ctx, const Grid3 &g)
+{
+    const SweepVariant &sv = sweep_variant(ctx->sweep);
+    return plan_group_steps(ctx->paired != 0, (g.nrows + sv.ti - 1) / sv.ti,
+                            (g.ncols + sv.tj - 1) / sv.tj);
+}
+
+int issue_steps3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o, int k0, int k1,
+                  hipStream_t st, const TileSkip *skip, const int *dphase)
 {
     if (!st) st = ctx->stream;
-    const SweepVariant &sv = sweep_variant(ctx->sweep);
     SweepArgs a = sweep_args(ctx, g, chunks, changed, skip);
     const int nti = a.nti, ntj = a.ntj;
     const bool skipping = a.tmod != nullptr;
-    a.o = o;
+    const bool paired = ctx->paired != 0;
     if (dphase) {
         a.phase = dphase;
         a.live = dphase + 2;
@@ -1051,19 +1068,33 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
     aa.active = adj;
     aa.nact = nadj;
     aa.nact_p8 = (nadj + 7) / 8 * 8;
-    const int s3 = o < 4 ? -1 : 1, s1 = (o & 2) ? 1 : -1, s2 = (o & 1) ? 1 : -1;
-    const int nd = nti + ntj - 1;
-    for (int D = D0; D < D1; D++) {
-        const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
-        if (skipping) af.epoch = aa.epoch = 1 + ((ecyc * 8 + o) * nd + D);
+    const int nd = nti + ntj - 1, lag = pair_lag(nti, ntj);
+    auto epoch = [&](int oo, int D) { return skipping ? 1 + ((ecyc * 8 + oo) * nd + D) : 0; };
+    for (int k = k0; k < k1; k++) {
+        const PlanStep p = plan_step(paired, nd, lag, o, k);
+        if (p.nparts == 0) continue;
+        const int o0 = p.o[0], D = p.D[0];
+        const int ntA = plan_diag_tiles(nti, ntj, D);
+        int nt = ntA;
+        af.o = aa.o = o0;
+        af.epoch = aa.epoch = epoch(o0, D);
+        af.ntA = aa.ntA = 0x7fffffff;
+        if (p.nparts == 2) {
+            af.ntA = aa.ntA = ntA;
+            af.DB = aa.DB = p.D[1];
+            af.oB = aa.oB = p.o[1];
+            af.epochB = aa.epochB = epoch(p.o[1], p.D[1]);
+            nt += plan_diag_tiles(nti, ntj, p.D[1]);
+        }
+        const int s1 = ord_s1(o0), s2 = ord_s2(o0), s3 = ord_s3(o0);
         if (nfwd) {
             time_begin(ctx, kFwdDiag, st);
-            HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, hi - lo + 1));
+            HIPCHK(launch_sweep3d(st, af, ctx->sweep, 0, g.uniform, D, s1, s2, s3, nt));
             time_end(ctx, st);
         }
         if (nadj) {
             time_begin(ctx, kAdjDiag, st);
-            HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, hi - lo + 1));
+            HIPCHK(launch_sweep3d(st, aa, ctx->sweep, 1, g.uniform, D, s1, s2, s3, nt));
             time_end(ctx, st);
         }
     }
@@ -1071,13 +1102,13 @@ int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, co
 }
 
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0, int o1,
-                  hipStream_t st, const TileSkip *skip, const int *dphase)
+                  int nfwd, const int *adj, int nadj, unsigned *changed, hipStream_t st,
+                  const TileSkip *skip, const int *dphase)
 {
-    const int nd = diagonals3d(ctx, g);
-    for (int o = o0; o < o1; o++) {
-        int rc = issue_diagonals3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, nd, st,
-                                   skip, dphase);
+    const int ns = group_steps3d(ctx, g), go = group_orderings3d(ctx);
+    for (int o = 0; o < 8; o += go) {
+        int rc = issue_steps3d(ctx, g, chunks, fwd, nfwd, adj, nadj, changed, o, 0, ns, st, skip,
+                               dphase);
         if (rc) return rc;
     }
     return PFAST_OK;

@@ -46,6 +46,7 @@ struct pfast_ctx {
     int nstreams = 1;
     int tile_skip = 1;  // exact tile skipping in the column-window sweeps
     int dataflow = 2;   // batched 3D cycles as dataflow launches: 0 never, 1 always, 2 auto
+    int paired = 1;     // per-diagonal launches pair orderings 2p and 2p+1 (plan.h)
     // batched 3D phases: sweep cycles issued between two convergence checks on the host
     // (a converged chunk retires on the device at the end of its cycle either way), and
     // whether those cycles are replayed from a captured hipGraph

@@ -10,7 +10,7 @@
 //     leave at cycle boundaries (continuous batching): every chunk sees exactly the
 //     sequence of full cycles pfast_shots3d_run gives it alone, so its traveltimes,
 //     lambda, gradient and misfit are bit-identical to a standalone run.
-//     A cycle is issued in slices of kSlice tile anti-diagonals with at most two
+//     A cycle is issued in slices of kSlice launch steps with at most two
 //     slices queued, so that the streams of other groups and of 2D requests, which
 //     may share a hardware queue with a large group, wait for milliseconds of its
 //     work rather than for a whole cycle.
@@ -43,7 +43,7 @@
 namespace {
 
 constexpr int kStreams2D = 2;
-constexpr int kSlice = 8;     // tile anti-diagonals per queued slice of a group cycle
+constexpr int kSlice = 8;     // launch steps (tile anti-diagonals) per queued slice of a cycle
 constexpr int kQueued = 2;    // slices of a group cycle queued at once
 constexpr int kMaxSrc2D = 1 << 16;  // per-request source limit of the 2D status staging
 
@@ -94,7 +94,7 @@ struct Group {
     bool dirty = true;
     hipEvent_t ev[kQueued] = {};  // end of the queued slices, by slice number mod kQueued
     bool inflight = false;        // a cycle is in progress
-    int o = 0, D = 0;             // next slice: ordering, first anti-diagonal
+    int o = 0, D = 0;             // next slice: launch group (first ordering), first step
     int issued = 0, done = 0, nfwd = 0, nadj = 0;
     // exact tile skipping: records by chunk id (cap x ntile), zeroed when a chunk
     // starts a phase; epochs from the group's cycle counter
@@ -389,17 +389,17 @@ void try_admit(pfast_server *S)
 int pump_cycle(Group &G)
 {
     hipStream_t st = G.ctx->stream;
-    const int nd = pfast::diagonals3d(G.ctx, G.g);
+    const int ns = pfast::group_steps3d(G.ctx, G.g), go = pfast::group_orderings3d(G.ctx);
     while (G.o < 8 && G.issued - G.done < kQueued) {
-        const int D1 = std::min(G.D + kSlice, nd);
+        const int D1 = std::min(G.D + kSlice, ns);
         const pfast::TileSkip skip{G.tmod, G.tclean, (int)G.cycle_no};
-        int rc = pfast::issue_diagonals3d(G.ctx, G.g, G.dtable, G.dact, G.nfwd, G.dact + G.nfwd,
-                                          G.nadj, G.dchanged, G.o, G.D, D1, nullptr,
-                                          G.tmod ? &skip : nullptr);
+        int rc = pfast::issue_steps3d(G.ctx, G.g, G.dtable, G.dact, G.nfwd, G.dact + G.nfwd,
+                                      G.nadj, G.dchanged, G.o, G.D, D1, nullptr,
+                                      G.tmod ? &skip : nullptr);
         if (rc) return rc;
         G.D = D1;
-        if (G.D == nd) {
-            G.o++;
+        if (G.D == ns) {
+            G.o += go;
             G.D = 0;
         }
         if (G.o == 8)

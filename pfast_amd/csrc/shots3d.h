@@ -103,14 +103,17 @@ struct TileSkip {
 // dphase: the device phase state (SweepArgs::phase) the launches read their cycle
 // number and live chunks from, or NULL (the cycle number in skip, every chunk live).
 int issue_cycle3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                  int nfwd, const int *adj, int nadj, unsigned *changed, int o0 = 0, int o1 = 8,
+                  int nfwd, const int *adj, int nadj, unsigned *changed, hipStream_t st = nullptr,
+                  const TileSkip *skip = nullptr, const int *dphase = nullptr);
+// A cycle's per-diagonal launches come in groups of group_orderings3d orderings (2 when
+// ctx pairs them, plan.h), group_steps3d launches each.  issue_steps3d: launch steps
+// [k0, k1) of the group starting at ordering o (same arguments otherwise).
+int group_orderings3d(const pfast_ctx *ctx);
+int group_steps3d(const pfast_ctx *ctx, const Grid3 &g);
+int issue_steps3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
+                  int nfwd, const int *adj, int nadj, unsigned *changed, int o, int k0, int k1,
                   hipStream_t st = nullptr, const TileSkip *skip = nullptr,
                   const int *dphase = nullptr);
-// Tile anti-diagonals [D0, D1) of ordering o (same arguments otherwise).
-int issue_diagonals3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const int *fwd,
-                      int nfwd, const int *adj, int nadj, unsigned *changed, int o, int D0, int D1,
-                      hipStream_t st = nullptr, const TileSkip *skip = nullptr,
-                      const int *dphase = nullptr);
 // Active chunks x tiles of one ordering up to which the automatic schedule takes the
 // dataflow launches: they keep the GPU busy where a launch per tile anti-diagonal
 // has too few tiles (measured on 201x201x101: 2.0x faster with 64 sources, 1.4x with

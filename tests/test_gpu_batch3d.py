@@ -190,6 +190,41 @@ def test_tile_skip_bitwise_identical(seed, shape, spacing):
     assert a[4] == b[4]
 
 
+@pytest.mark.parametrize("variant", [0, 5, 6, 8])
+def test_paired_launches_bitwise_identical(variant):
+    """Per-diagonal launches that pair orderings 2p and 2p+1 (the second trailing by
+    ntj + 1 tile diagonals) sweep every tile with the values the one-ordering launches
+    give it: fields, cycle counts, gradient and misfit are the same bits, with and
+    without tile skipping, for tile shapes with different ntj."""
+    import pfast_amd
+    c = cases.case3d(47 + variant, shape=(19, 35, 41), spacing=(25.0, 25.0, 25.0), nsrc=11,
+                     nrec_per=12)
+    outs = {}
+    for skip in (False, True):
+        for paired in (False, True):
+            ctx = pfast_amd.Context(0)
+            ctx.set_sweep(variant)
+            ctx.set_dataflow(0)
+            ctx.set_tile_skip(skip)
+            ctx.set_paired(paired)
+            m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+            sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+            sh.run(adjoint=True, MAXSwp=100)
+            g, E = sh.gradient()
+            outs[skip, paired] = (np.stack([sh.field(s) for s in range(11)]),
+                                  np.stack([sh.field(s, adjoint=True) for s in range(11)]),
+                                  sh.cycles(), g, E)
+            sh.close()
+            m.close()
+            ctx.close()
+    ref = outs[False, False]
+    print(f"variant {variant} cycles {ref[2].tolist()}", flush=True)
+    for key, out in outs.items():
+        assert all(np.array_equal(cases.bits(x), cases.bits(y))
+                   for x, y in zip(out[:4], ref[:4])), key
+        assert out[4] == ref[4], key
+
+
 def test_sweep_variant_rejects_unknown():
     import pfast_amd
     ctx = pfast_amd.Context(0)
