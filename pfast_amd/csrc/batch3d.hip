@@ -126,7 +126,7 @@ struct ColLane {
     int c, li, lj, pipj;
     bool valid;
     bool uxm, uxp, uym, uyp;   // that neighbour is a halo node (read from HBM)
-    int rxm, rxp, rym, ryp;    // LDS indices of the neighbours' published values
+    int rxm, rxp, rym, ryp;    // LDS indices of the neighbours' values in a publish block
     unsigned oI;               // byte offset of level 0 of the own column
     unsigned oF;               // byte offset of level 0 in the cell-time array
     unsigned step, stepF;      // byte step per level (two's complement when s3 < 0)
@@ -155,12 +155,18 @@ __device__ __forceinline__ ColLane<TI, TJ, C> make_col_lane(const SweepArgs &a, 
     l.uym = l.valid && eym && t.i0 > 0;
     l.uyp = l.valid && !l.uym && eyp && t.i0 + t.TIe < a.nrows;
     // upstream neighbour (one step ahead): its "m-1" slot [0, NT); downstream: its
-    // "m+1" slot [NT, 2NT).  Idle threads read their own slot (unused).
+    // "m+1" slot [NT, 2NT); a halo node: its slot among the halo values published for
+    // the step after the INF slot, faces x- (TI rows), x+ (TI), y- (TJ columns), y+ (TJ),
+    // C sources per face position (HaloStage).  Idle threads read their own slot (unused).
+    constexpr int HALO = INFSLOT + 1;
     const bool ok = l.valid;
-    l.rxm = !ok ? tid : exm ? INFSLOT : (s2 > 0 ? 0 : NT) + tid - C;
-    l.rxp = !ok ? tid : exp_ ? INFSLOT : (s2 > 0 ? NT : 0) + tid + C;
-    l.rym = !ok ? tid : eym ? INFSLOT : (s1 > 0 ? 0 : NT) + tid - C * TJ;
-    l.ryp = !ok ? tid : eyp ? INFSLOT : (s1 > 0 ? NT : 0) + tid + C * TJ;
+    l.rxm = !ok ? tid : l.uxm ? HALO + l.li * C + l.c : exm ? INFSLOT : (s2 > 0 ? 0 : NT) + tid - C;
+    l.rxp = !ok ? tid : l.uxp ? HALO + (TI + l.li) * C + l.c
+                      : exp_ ? INFSLOT : (s2 > 0 ? NT : 0) + tid + C;
+    l.rym = !ok ? tid : l.uym ? HALO + (2 * TI + l.lj) * C + l.c
+                      : eym ? INFSLOT : (s1 > 0 ? 0 : NT) + tid - C * TJ;
+    l.ryp = !ok ? tid : l.uyp ? HALO + (2 * TI + TJ + l.lj) * C + l.c
+                      : eyp ? INFSLOT : (s1 > 0 ? NT : 0) + tid + C * TJ;
     const unsigned plane = (unsigned)a.nrows * a.ncols, pC = plane * C;
     l.node = l.valid ? (unsigned)gi * a.ncols + gj : 0u;
     const unsigned k0 = s3 > 0 ? 0u : (unsigned)(a.nz - 1);
@@ -341,36 +347,41 @@ __device__ __forceinline__ void fstore(rsrc_t r, unsigned off, float v)
     else bstore(r, off, v);
 }
 
-// ------------------------------------------------------------ halo staging
+// ------------------------------------------------------------ halo values
 // A tile's four halo faces (the columns beside its x edges, the rows beside its y
 // edges) are read by the lanes on those edges only: with a load per lane, a wave of
 // an edge row would carry a second full row of loads and every wave a sliver of x
 // halo per step, and the block waits at each step for its slowest wave.  Instead
 // every thread of the block loads a share of the faces for a whole batch of L steps
 // (full-wave loads, consecutive lanes on consecutive sources of one face node), a
-// batch ahead like the column's own values; at the batch start it writes them into an
-// LDS stage, and after the first step's barrier each edge lane reads its L values from
-// there.  Stage slot of step u, face f, position q along the face, source c:
-//   u * HSU + face offset + q * C + c,  faces x- (TI rows), x+ (TI), y- (TJ columns), y+ (TJ).
-// A value stands for the level the consuming edge lane works on at that step, so the
-// loader takes the consumer's sweep position into its address.
+// batch ahead like the column's own values.  At step u the threads holding that
+// step's values write them into the halo slots of the step's publish block, beside
+// the levels the lanes publish, and an edge lane reads its halo neighbour there like
+// any other neighbour (its neighbour index points at the slot), so no lane selects
+// between a published and a halo value.  Value s = u * HSU + face offset + q * C + c
+// (step u, face f, position q along the face, source c; faces x- (TI rows), x+ (TI),
+// y- (TJ columns), y+ (TJ)) is loaded by thread s % NT and goes to halo slot
+// s % HSU.  A value stands for the level the consuming edge lane works on at that
+// step, so the loader takes the consumer's sweep position into its address.
 template <int TI, int TJ, int C, int L>
 struct HaloStage {
     static constexpr int NT = TI * TJ * C;
-    static constexpr int HSU = 2 * (TI + TJ) * C;  // stage slots per step
-    static constexpr int NS = L * HSU;             // per batch (one parity)
-    static constexpr int K = (NS + NT - 1) / NT;   // slots loaded per thread
-    unsigned addr[K];  // byte offset of the slot's value at batch start step 0
-    bool use[K];       // the slot feeds an edge lane that reads a halo in the domain
-    int xs, ys;        // this lane's x / y face slot at step 0 (edge lanes)
+    static constexpr int HSU = 2 * (TI + TJ) * C;  // halo values per step
+    static constexpr int NS = L * HSU;             // per batch
+    static constexpr int K = (NS + NT - 1) / NT;   // values loaded per thread
+    // publish block of a step parity: [m-1 | m+1][thread], INF slot, HSU halo slots
+    static constexpr int PB = 2 * NT + 1 + HSU;
+    static_assert(HSU % 64 == 0 && NT % 64 == 0, "the values of a wave are for one step");
+    unsigned addr[K];  // byte offset of the value at batch start step 0
+    bool use[K];       // the value feeds an edge lane that reads a halo in the domain
+    int uk[K];         // step of the batch the value is for (wave-uniform; >= L: none)
+    int rk[K];         // its halo slot
 };
 
 template <int TI, int TJ, int C, int L>
 __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepArgs &a,
-                                                                   const TileCtx &t,
-                                                                   const ColLane<TI, TJ, C> &l,
-                                                                   int s1, int s2, int s3,
-                                                                   unsigned rec)
+                                                                   const TileCtx &t, int s1,
+                                                                   int s2, int s3, unsigned rec)
 {
     using H = HaloStage<TI, TJ, C, L>;
     H h;
@@ -401,10 +412,9 @@ __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepAr
         const unsigned node = in ? (unsigned)gi * a.ncols + (unsigned)gj : 0u;
         h.use[k] = in;
         h.addr[k] = 4u * rec * (k0 * pC + node * C + (unsigned)c) + (unsigned)(u - pipj) * st;
+        h.uk[k] = __builtin_amdgcn_readfirstlane(u);
+        h.rk[k] = r;
     }
-    const int lc = l.c;
-    h.xs = (l.uxm ? 0 : TI * C) + l.li * C + lc;
-    h.ys = (l.uym ? 2 * TI * C : (2 * TI + TJ) * C) + l.lj * C + lc;
     return h;
 }
 
@@ -414,14 +424,16 @@ __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepAr
 // memory operations and publish the task's progress every batch (flow.h).
 template <int TI, int TJ, int C, int L, bool UNI, bool FLOW>
 __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int s3, int six, float *pub, float *hstage,
-                                         int *any_change, FlowWait *fw, unsigned *self)
+                                         int s3, int six, float *pub, int *any_change,
+                                         FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
     static_assert(C % 4 == 0, "a node's sources fill whole DPP quads");
-    // per step parity: [m-1 | m+1][thread], then an INF slot (domain-edge neighbour)
-    constexpr int PB = 2 * NT + 1;
+    // per step parity: [m-1 | m+1][thread], an INF slot (domain-edge neighbour), the
+    // step's halo values
+    using HS = HaloStage<TI, TJ, C, L>;
+    constexpr int PB = HS::PB;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
@@ -466,17 +478,16 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
         stF = s3 > 0 ? 4u * tplane : 0u - 4u * tplane;
     }
     // The vector-memory path bounds the sweep's throughput, so: the halo faces are
-    // loaded by the whole block into an LDS stage (HaloStage), and the cell time, one
-    // value per node, is loaded for four levels at once, lane c of each DPP quad of the
-    // node's lanes taking level 4g + c%4; step u broadcasts it within the quad.
+    // loaded by the whole block and published step by step (HaloStage), and the cell
+    // time, one value per node, is loaded for four levels at once, lane c of each DPP
+    // quad of the node's lanes taking level 4g + c%4; step u broadcasts it within the
+    // quad.
     constexpr int FQ = (L + 3) / 4;
-    using HS = HaloStage<TI, TJ, C, L>;
-    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, l, s1, s2, s3, 1u);
+    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, s1, s2, s3, 1u);
     const unsigned oFc = oF + (unsigned)(l.c & 3) * stF;
-    float hl[HS::K];  // this thread's stage values of the next batch
     // queue of one batch: own column two levels ahead (m+2, INF outside the column),
-    // cell time of levels m .. m+4*FQ-1; and the stage values of that batch
-    auto load_batch = [&](int pb, float *ow, float *fv) {
+    // cell time of levels m .. m+4*FQ-1; and this thread's halo values of that batch
+    auto load_batch = [&](int pb, float *ow, float *fv, float *hl) {
         const int mb = pb - l.pipj;
         const unsigned dm = (unsigned)mb * st, dmf = (unsigned)mb * stF;
 #pragma unroll
@@ -499,31 +510,27 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     float vprev = INF;
     float vcur = l.valid && m0 == 0 ? fload<FLOW>(RF, l.oI) : INF;
     float vnext = l.valid && m0 >= -1 ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
-    float qa[2][L], qb[2][L];
-    load_batch(0, qa[0], qa[1]);
+    float qa[2][L], qb[2][L], ha[HS::K], hb[HS::K];
+    load_batch(0, qa[0], qa[1], ha);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();  // INF slot visible
 
     bool changed = false;
-    float sval[L], hx[L], hy[L];
+    float sval[L];
     bool chg[L];
 #pragma unroll
     for (int u = 0; u < L; u++) {
         sval[u] = 0.0f;
         chg[u] = false;
     }
-    // one batch of L steps from queue q (loaded a batch earlier) and stage parity par;
-    // issues the loads of the following batch into nq first
-    auto batch = [&](int pb, int par, float (*q)[L], float (*nq)[L]) {
+    // one batch of L steps from queue q and halo values h (loaded a batch earlier);
+    // issues the loads of the following batch into nq, nh first
+    auto batch = [&](int pb, float (*q)[L], const float *h, float (*nq)[L], float *nh) {
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
             flow_prefetch(*fw);
         }
-        float *Hs = hstage + par * HS::NS;
-#pragma unroll
-        for (int k = 0; k < HS::K; k++)
-            if (HS::NS % HS::NT == 0 || tid + k * HS::NT < HS::NS) Hs[tid + k * HS::NT] = hl[k];
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
@@ -535,28 +542,23 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             for (int u = 0; u < L; u++)
                 if (chg[u]) fstore<FLOW>(RF, l.oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pbo + L, nq[0], nq[1]);
+        load_batch(pbo + L, nq[0], nq[1], nh);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
             float *P = pub + (u & 1) * PB;
             P[tid] = vprev;
             P[NT + tid] = vnext;
+#pragma unroll
+            for (int k = 0; k < HS::K; k++)
+                if (hs.uk[k] == u) P[2 * NT + 1 + hs.rk[k]] = h[k];
             lds_barrier();
             if constexpr (FLOW) {
                 // every wave has waited for its stores of steps < pb - L
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
             }
-            if (u == 0) {  // the batch's halo values of this lane's faces
-#pragma unroll
-                for (int v = 0; v < L; v++) {
-                    hx[v] = Hs[hs.xs + v * HS::HSU];
-                    hy[v] = Hs[hs.ys + v * HS::HSU];
-                }
-            }
-            const float pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            const float xa = fminf(l.uxm ? hx[u] : pxm, l.uxp ? hx[u] : pxp);
-            const float ya = fminf(l.uym ? hy[u] : pym, l.uyp ? hy[u] : pyp);
+            const float xa = fminf(P[l.rxm], P[l.rxp]);
+            const float ya = fminf(P[l.rym], P[l.ryp]);
             const float za = fminf(vprev, vnext);
             const float fval = quad_bcast(q[1][u >> 2], u);
             float nv = vcur;
@@ -580,9 +582,9 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     // loop ends after the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
     for (int pb = 0;; pb += 2 * L) {
-        batch(pb, 0, qa, qb);
+        batch(pb, qa, ha, qb, hb);
         if (pb + L >= pend) break;
-        batch(pb + L, 1, qb, qa);
+        batch(pb + L, qb, hb, qa, ha);
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -603,17 +605,14 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
 template <int TI, int TJ, int C, int L, bool UNI>
 __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s1, int s2, int s3)
 {
-    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
-    __shared__ float pub[2 * PB];
-    __shared__ float hstage[2 * HaloStage<TI, TJ, C, L>::NS];
+    __shared__ float pub[2 * HaloStage<TI, TJ, C, L>::PB];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, hstage, &any_change, nullptr,
-                                       nullptr);
+    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr, nullptr);
 }
 
 // Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
@@ -623,13 +622,15 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS, bool FLOW>
 __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int six, float2 *pub, float2 *hstage, int *any_change,
-                                         FlowWait *fw, unsigned *self)
+                                         int six, float2 *pub, int *any_change, FlowWait *fw,
+                                         unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     constexpr int s3 = S3POS ? 1 : -1;
     static_assert(L % 2 == 0, "the step parity must be static inside a batch");
-    constexpr int PB = 2 * NT + 1;  // per step parity: [m-1 | m+1][thread], INF slot
+    // per step parity: [m-1 | m+1][thread], INF slot, the step's halo pairs
+    using HS = HaloStage<TI, TJ, C, L>;
+    constexpr int PB = HS::PB;
     const ColLane<TI, TJ, C> l = make_col_lane<TI, TJ, C>(a, t, s1, s2, s3);
     const int tid = threadIdx.x;
     const int nz = a.nz;
@@ -652,12 +653,11 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         *any_change = 0;
     }
 
-    // the halo faces come through the block's LDS stage (HaloStage)
-    using HS = HaloStage<TI, TJ, C, L>;
-    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, l, s1, s2, s3, 2u);
-    float2 hl[HS::K];  // this thread's stage values of the next batch
+    // the halo faces are loaded by the whole block and published step by step
+    // (HaloStage)
+    const HS hs = make_halo_stage<TI, TJ, C, L>(a, t, s1, s2, s3, 2u);
     auto ld2 = [&](unsigned o) { return fload2<FLOW>(RLT, o); };
-    auto load_batch = [&](int pb, float2 *ow) {
+    auto load_batch = [&](int pb, float2 *ow, float2 *hl) {
         const unsigned dm = (unsigned)(pb - l.pipj) * st;
 #pragma unroll
         for (int u = 0; u < L; u++) ow[u] = ld2(oI + dm + (unsigned)(u + 2) * st);
@@ -674,8 +674,8 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     float2 vprev = make_float2(0.0f, 0.0f);
     float2 vcur = l.valid && m0 == 0 ? ld2(oI) : make_float2(0.0f, 0.0f);
     float2 vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
-    float2 qa[1][L], qb[1][L], hx[L], hy[L];
-    load_batch(0, qa[0]);
+    float2 qa[1][L], qb[1][L], ha[HS::K], hb[HS::K];
+    load_batch(0, qa[0], ha);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();
 
@@ -687,16 +687,12 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         sval[u] = 0.0f;
         chg[u] = false;
     }
-    auto batch = [&](int pb, int par, float2 (*q)[L], float2 (*nq)[L]) {
+    auto batch = [&](int pb, float2 (*q)[L], const float2 *h, float2 (*nq)[L], float2 *nh) {
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
             flow_prefetch(*fw);
         }
-        float2 *Hs = hstage + par * HS::NS;
-#pragma unroll
-        for (int k = 0; k < HS::K; k++)
-            if (HS::NS % HS::NT == 0 || tid + k * HS::NT < HS::NS) Hs[tid + k * HS::NT] = hl[k];
         // an opaque copy of the batch start for the addresses: otherwise the compiler
         // keeps every per-step address of the queues in a register of its own
         int pbo = pb;
@@ -708,31 +704,24 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             for (int u = 0; u < L; u++)
                 if (chg[u]) fstore<FLOW>(RLT, oI + d + (unsigned)u * st, sval[u]);
         }
-        load_batch(pbo + L, nq[0]);
+        load_batch(pbo + L, nq[0], nh);
 #pragma unroll
         for (int u = 0; u < L; u++) {
             const int p = pb + u;
             float2 *P = pub + (u & 1) * PB;
             P[tid] = vprev;
             P[NT + tid] = vnext;
+#pragma unroll
+            for (int k = 0; k < HS::K; k++)
+                if (hs.uk[k] == u) P[2 * NT + 1 + hs.rk[k]] = h[k];
             lds_barrier();
             if constexpr (FLOW) {
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
             }
-            if (u == 0) {  // the batch's halo pairs of this lane's faces
-#pragma unroll
-                for (int v = 0; v < L; v++) {
-                    hx[v] = Hs[hs.xs + v * HS::HSU];
-                    hy[v] = Hs[hs.ys + v * HS::HSU];
-                }
-            }
             // branch-free: the (lambda, T) pairs of the neighbours come in one LDS read
-            // each, the update is computed in every lane and selected afterwards
-            const float2 pxm = P[l.rxm], pxp = P[l.rxp], pym = P[l.rym], pyp = P[l.ryp];
-            const float2 xm = l.uxm ? hx[u] : pxm;
-            const float2 xp = l.uxp ? hx[u] : pxp;
-            const float2 ym = l.uym ? hy[u] : pym;
-            const float2 yp = l.uyp ? hy[u] : pyp;
+            // each (halo pairs too), the update is computed in every lane and selected
+            // afterwards
+            const float2 xm = P[l.rxm], xp = P[l.rxp], ym = P[l.rym], yp = P[l.ryp];
             // depth k-1 / k+1 by absolute depth (levels m-1 / m+1 are depths
             // k-s3 / k+s3): every ordering sums the terms in one order, so a
             // stationary node stays bitwise stationary
@@ -772,9 +761,9 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
     for (int pb = 0;; pb += 2 * L) {
-        batch(pb, 0, qa, qb);
+        batch(pb, qa, ha, qb, hb);
         if (pb + L >= pend) break;
-        batch(pb + L, 1, qb, qa);
+        batch(pb + L, qb, hb, qa, ha);
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -795,16 +784,14 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
 {
-    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
-    __shared__ float2 pub[2 * PB];
-    __shared__ float2 hstage[2 * HaloStage<TI, TJ, C, L>::NS];
+    __shared__ float2 pub[2 * HaloStage<TI, TJ, C, L>::PB];
     __shared__ int any_change;
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, hstage, &any_change, nullptr,
+    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr,
                                               nullptr);
 }
 
@@ -966,9 +953,7 @@ template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void fwd3d_flow(SweepArgs a, FlowArgs f)
 {
     flow_phase(f);
-    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
-    __shared__ float pub[2 * PB];
-    __shared__ float hstage[2 * HaloStage<TI, TJ, C, L>::NS];
+    __shared__ float pub[2 * HaloStage<TI, TJ, C, L>::PB];
     __shared__ int any_change;
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
@@ -982,7 +967,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     fwd_tile<TI, TJ, C, L, UNI, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), S3POS ? 1 : -1,
-                                      six, pub, hstage, &any_change, &fw, self);
+                                      six, pub, &any_change, &fw, self);
     flow_finish(f, self);
 }
 
@@ -990,9 +975,7 @@ template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) void adj3d_flow(SweepArgs a, FlowArgs f)
 {
     flow_phase(f);
-    constexpr int NT = TI * TJ * C, PB = 2 * NT + 1;
-    __shared__ float2 pub[2 * PB];
-    __shared__ float2 hstage[2 * HaloStage<TI, TJ, C, L>::NS];
+    __shared__ float2 pub[2 * HaloStage<TI, TJ, C, L>::PB];
     __shared__ int any_change;
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
@@ -1006,7 +989,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
-                                             hstage, &any_change, &fw, self);
+                                             &any_change, &fw, self);
     flow_finish(f, self);
 }
 
@@ -1279,9 +1262,9 @@ namespace pfast {
 // Sweep kernel variants (tile rows x columns, batch depth of the forward / adjoint
 // kernel), all of the column-window schedule.  Variant 0 is the default; the others
 // stay selectable for measurement and for the autotuner.  The adjoint keeps a
-// (lambda, T) pair per queued value, so a shallower batch keeps its register count at
-// full occupancy (8x8 tiles: 54 VGPRs at L2 against 84 at L4, 8 against 5 waves per
-// SIMD).  Tiles of 1024 threads (8x16, 16x8) and batch depth 8 measured 1.4-2.8x slower
+// (lambda, T) pair per queued value, so its register count grows fast with the batch
+// depth (8x8 tiles: 46 VGPRs at L2, 64 at L4, 83 at L6, which leaves 6 waves per SIMD
+// instead of 8).  Tiles of 1024 threads (8x16, 16x8) and batch depth 8 measured 1.4-2.8x slower
 // on the 400-source workload and are not built.
 #define PFAST_VARIANTS(X)                                                                 \
     X(8, 8, 4, 2) X(8, 8, 4, 4) X(8, 8, 6, 6) X(8, 8, 2, 2) X(8, 8, 6, 2) X(8, 4, 4, 4)     \
