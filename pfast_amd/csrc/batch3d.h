@@ -28,6 +28,9 @@ struct SweepArgs {
     int nact, nact_p8;    // active chunks; grid stride (multiple of 8 = XCD count)
     int nrows, ncols, nz, nti, ntj;
     float hx, hy, hz, wx, wy, wz;  // spacings (x=cols, y=rows, z=depth), 1/h^2
+    // uniform spacing: the adjoint's unweighted denominator den passes the threshold
+    // den * wx > 1e-9 (FP32 product) exactly when den > adj_thr
+    float adj_thr;
     // Exact tile skipping (column-window kernels; NULL: off).  Per (chunk, tile):
     // tmod[5] = epochs of the last sweeps that changed a node of the tile ([0]) and
     // of its boundary faces x-, x+, y-, y+ ([1..4]: column j = 0, column j = last,

@@ -421,8 +421,12 @@ __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepAr
 // ======================================================= column-window tiles
 // One tile sweep of the forward column window (the body of fwd3d_col and of each
 // task of fwd3d_flow).  FLOW: wait on the task's dependencies before each batch's
-// memory operations and publish the task's progress every batch (flow.h).
-template <int TI, int TJ, int C, int L, bool UNI, bool FLOW>
+// memory operations and publish the task's progress every batch (flow.h).  BOX: the
+// chunk's sources hold fixed boxes (half-width R > 0) that the sweep must leave alone;
+// with R = 0 the source node alone is fixed, and no local solve can lower its 0 (each
+// is at least the smallest neighbour value, and traveltimes are never negative), so
+// the sweep needs no test for it.
+template <int TI, int TJ, int C, int L, bool UNI, bool FLOW, bool BOX>
 __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
                                          int s3, int six, float *pub, int *any_change,
                                          FlowWait *fw, unsigned *self)
@@ -459,7 +463,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     const int R = a.chunks[t.chunk].srcinit;
     const unsigned pfn = 2u * (unsigned)R;
     int pfl = 0x3fffffff;
-    {
+    if constexpr (BOX) {
         const int sn = a.chunks[t.chunk].srcnode[l.c];
         if (l.valid && sn >= 0) {
             const unsigned sc = (unsigned)sn % plane;
@@ -563,7 +567,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             const float fval = quad_bcast(q[1][u >> 2], u);
             float nv = vcur;
             bool c = false;
-            if ((unsigned)(p - plo) < (unsigned)nz && (unsigned)(p - pfl) > pfn) {
+            if ((unsigned)(p - plo) < (unsigned)nz && (!BOX || (unsigned)(p - pfl) > pfn)) {
                 float qv;
                 if (UNI) qv = eik3d_f32_uniform(xa, ya, za, fval);
                 else qv = eik3d_f32_general(xa, ya, za, fval, a.hx, a.hy, a.hz, a.wx, a.wy, a.wz);
@@ -612,7 +616,12 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
     phase_epoch(a);
-    fwd_tile<TI, TJ, C, L, UNI, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr, nullptr);
+    if (a.chunks[t.chunk].srcinit > 0)
+        fwd_tile<TI, TJ, C, L, UNI, false, true>(a, t, s1, s2, s3, six, pub, &any_change, nullptr,
+                                                 nullptr);
+    else
+        fwd_tile<TI, TJ, C, L, UNI, false, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr,
+                                                  nullptr);
 }
 
 // Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
@@ -733,19 +742,20 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             // any stencil traveltime at INF: one compare of the maximum (three max3)
             const float tmax = fmaxf(fmaxf(fmaxf(txm, txp), tym),
                                      fmaxf(fmaxf(typ, tzm), fmaxf(tzp, tc)));
-            float num = 0.0f, den = 0.0f;
-            adj_acc<UNI>(txp - tc, xp.x, wx, num, den);
+            float num, den;
+            adj_first<UNI>(txp - tc, xp.x, wx, num, den);
             adj_acc<UNI>(txm - tc, xm.x, wx, num, den);
             adj_acc<UNI>(typ - tc, yp.x, wy, num, den);
             adj_acc<UNI>(tym - tc, ym.x, wy, num, den);
             adj_acc<UNI>(tzp - tc, zp.x, wz, num, den);
             adj_acc<UNI>(tzm - tc, zm.x, wz, num, den);
-            // the reference's threshold applies to the weighted denominator
-            const float denw = UNI ? den * wx : den;
+            // the reference's threshold applies to the weighted denominator: with
+            // uniform spacing den * wx > 1e-9 exactly when den > adj_thr
+            const float thr = UNI ? a.adj_thr : 1e-9f;
             // computed in every lane and selected: no branch around the update
             float r = num * frcp(den);
             asm volatile("" : "+v"(r));
-            const float nw = tmax < PFAST_INF_F && denw > 1e-9f ? r : 0.0f;
+            const float nw = tmax < PFAST_INF_F && den > thr ? r : 0.0f;
             const bool act = p >= plo && p < phi && !signbit(vcur.y);
             const bool c = act && nw != vcur.x;
             const float nl = act ? nw : vcur.x;
@@ -966,8 +976,12 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_task<TI, TJ>(a, f, s, t, at, fw, self);
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
-    fwd_tile<TI, TJ, C, L, UNI, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), S3POS ? 1 : -1,
-                                      six, pub, &any_change, &fw, self);
+    if (at.chunks[t.chunk].srcinit > 0)
+        fwd_tile<TI, TJ, C, L, UNI, true, true>(at, t, pflow::s1_of(o), pflow::s2_of(o),
+                                                S3POS ? 1 : -1, six, pub, &any_change, &fw, self);
+    else
+        fwd_tile<TI, TJ, C, L, UNI, true, false>(at, t, pflow::s1_of(o), pflow::s2_of(o),
+                                                 S3POS ? 1 : -1, six, pub, &any_change, &fw, self);
     flow_finish(f, self);
 }
 

@@ -967,6 +967,11 @@ SweepArgs sweep_args(const pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunk
     a.wx = (float)(1.0 / (g.dx * g.dx));
     a.wy = (float)(1.0 / (g.dy * g.dy));
     a.wz = (float)(1.0 / (g.dz * g.dz));
+    // the largest t with fl(t * wx) <= 1e-9 (the product is monotone in t)
+    float thr = 1e-9f / a.wx;
+    while (thr > 0.0f && thr * a.wx > 1e-9f) thr = std::nextafter(thr, 0.0f);
+    while (std::nextafter(thr, INFINITY) * a.wx <= 1e-9f) thr = std::nextafter(thr, INFINITY);
+    a.adj_thr = thr;
     const bool skipping = skip && sv.kind == 0;
     a.tmod = skipping ? skip->tmod : nullptr;
     a.tclean = skipping ? skip->tclean : nullptr;

@@ -162,8 +162,8 @@ struct Fast32 {
     {
         const float wx = (float)g.wx, wz = (float)g.wz;
         const float tmax = fmaxf(fmaxf(fmaxf(tc, txp), fmaxf(txm, tzp)), tzm);
-        float num = 0.0f, den = 0.0f;
-        adj_acc<false>(txp - tc, lxp, wx, num, den);
+        float num, den;
+        adj_first<false>(txp - tc, lxp, wx, num, den);
         adj_acc<false>(txm - tc, lxm, wx, num, den);
         adj_acc<false>(tzp - tc, lzp, wz, num, den);
         adj_acc<false>(tzm - tc, lzm, wz, num, den);

@@ -179,6 +179,19 @@ __device__ __forceinline__ float eik2d_f32(float a, float b, float s, float dx, 
 // lambdas, not even through the sign of a zero (a -0 there would couple a node to its
 // downwind neighbours and let its sign bit oscillate between sweeps).  Uniform spacing
 // (UNI): every w is the same and cancels in num / den; it is left out.
+// The first term sets den (a zero den of either sign fails the threshold alike) and
+// adds num's product to +0.
+template <bool UNI>
+__device__ __forceinline__ void adj_first(float d, float ln, float w, float &num, float &den)
+{
+    if (UNI) {
+        den = fmaxf(-d, 0.0f);
+        num = fmaf(fmaxf(d, 0.0f), ln, 0.0f);
+    } else {
+        den = w * fmaxf(-d, 0.0f);
+        num = fmaf(w * fmaxf(d, 0.0f), ln, 0.0f);
+    }
+}
 template <bool UNI>
 __device__ __forceinline__ void adj_acc(float d, float ln, float w, float &num, float &den)
 {
