@@ -95,6 +95,10 @@ def cpu_baseline(cfg, nthreads, nsamp, maxswp):
                        f"block), wall {wall:.1f} s")
 
 
+def tile_skip_mode(args):
+    return "off" if args.no_tile_skip else args.tile_skip
+
+
 def measured_traffic(kernel, sweep_name, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (profiles/traffic.json, copy-calibrated), when they were taken
@@ -108,7 +112,8 @@ def measured_traffic(kernel, sweep_name, args):
         return None
     if (t.get("sweep") != sweep_name or t.get("nsrc") != args.nsrc or t.get("nrec") != args.nrec
             or t.get("grid") != [101, 201, 201]
-            or t.get("tile_skip", False) != (not args.no_tile_skip)
+            or t.get("tile_skip", False) != (tile_skip_mode(args) != "off")
+            or t.get("start_levels", False) != (tile_skip_mode(args) == "levels")
             or t.get("paired", False) != (args.paired == "on")):
         return None
     return t.get("bytes_per_launch", {}).get(kernel)
@@ -326,8 +331,11 @@ def main():
     ap.add_argument("--paired", choices=["on", "off"], default="on",
                     help="per-diagonal launches pair orderings 2p and 2p+1, the second "
                          "trailing by ntj + 1 tile diagonals (same results)")
-    ap.add_argument("--no-tile-skip", action="store_true",
-                    help="sweep every tile (exact tile skipping off; same results)")
+    ap.add_argument("--tile-skip", choices=["off", "tiles", "levels"], default="levels",
+                    help="exact tile skipping: off, whole tiles, or whole tiles and start "
+                         "levels (a sweep starts at the first level whose inputs changed); "
+                         "same results")
+    ap.add_argument("--no-tile-skip", action="store_true", help="same as --tile-skip off")
     ap.add_argument("--streams", type=int, default=3,
                     help="streams a batched cycle splits its source chunks over in the timed "
                          "steps (1..4); the per-launch roofline step always uses one")
@@ -376,7 +384,7 @@ def main():
         f"{args.nsrc} sources x {args.nrec} receivers")
     ctx = pfast_amd.Context(0)
     ctx.set_streams(args.streams)
-    ctx.set_tile_skip(not args.no_tile_skip)
+    ctx.set_tile_skip({"off": 0, "tiles": 1, "levels": 2}[tile_skip_mode(args)])
     ctx.set_paired(args.paired == "on")
     ctx.set_dataflow({"auto": 2, "dataflow": 1, "diagonal": 0}[args.schedule])
     ctx.set_sweep(args.sweep)
@@ -517,7 +525,7 @@ def main():
                    "schedule": args.schedule, "check_interval": args.check_interval,
                    "graphs": args.graphs, "autotuned": bool(args.autotune),
                    "streams": args.streams, "paired": args.paired,
-                   "tile_skip": not args.no_tile_skip},
+                   "tile_skip": tile_skip_mode(args)},
         "roofline": roofline,
         "graphs": graphs_ab,
         "autotune": tune,

@@ -171,9 +171,11 @@ class Context:
         (bit-identical results)."""
         check(lib().pfast_ctx_set_sweep2d(self.h, int(variant)))
 
-    def set_tile_skip(self, enable=True):
-        """Exact tile skipping of the batched 3D sweeps (default on; same results)."""
-        check(lib().pfast_ctx_set_tile_skip(self.h, 1 if enable else 0))
+    def set_tile_skip(self, mode=True):
+        """Exact tile skipping of the batched 3D sweeps: False/0 off, 1 whole tiles,
+        True/2 (default) whole tiles and start levels (same results)."""
+        m = 2 if mode is True else 0 if mode is False else int(mode)
+        check(lib().pfast_ctx_set_tile_skip(self.h, m))
 
     def set_paired(self, enable=True):
         """Pair orderings 2p and 2p+1 of a batched 3D cycle in shared per-diagonal

@@ -21,6 +21,8 @@ struct ChunkRef {
                                // the source node alone, as the reference seeds it)
 };
 
+constexpr int kTileClean = 16;  // tclean ints per (chunk, tile): 8 epochs, 8 start levels
+
 struct SweepArgs {
     const ChunkRef *chunks;  // indexed by chunk id
     const int *active;       // chunk ids still sweeping
@@ -39,6 +41,17 @@ struct SweepArgs {
     // it, so a tile whose clean sweep in this ordering is newer than every change of
     // itself and of those four neighbour faces would see the same inputs again and
     // change nothing: skipped.
+    // Start levels (starts != 0, per-diagonal kernels): tclean[8 + o] = the first
+    // level (in ordering o's sweep order) from which o's next sweep of the tile can
+    // differ from its last one.  Below it every input of every update is the value
+    // the last sweep saw, so the sweep reproduces the stored values there and starts
+    // at that level; nz or more: nothing differs, skipped.  The last sweep in o sets
+    // it to its own first changed level less one (a node's update reads the levels
+    // beside it), and every change lowers it for the other orderings of the tile (by
+    // the same margin) and, on a face, for all orderings of the neighbour across it
+    // (to the changed level: a halo node is read at its own level only).  The
+    // dataflow kernels do not keep them: a dataflow cycle resets its chunks' start
+    // levels to 0 afterwards (launch_starts_reset).  kTileClean ints per (chunk, tile).
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
     // A paired launch (plan.h) sweeps two orderings that differ in the column direction
@@ -52,6 +65,7 @@ struct SweepArgs {
     const int *phase;
     const int *live;
     int ep_cycle;
+    int starts;  // start levels kept and used (per-diagonal kernels; see tclean)
     // node-sweeps of the tiles actually swept (columns x depth x real sources), added up
     // on the device while profiling (NULL: not counted)
     unsigned long long *swept;
@@ -85,6 +99,8 @@ struct SweepVariant {
 namespace pfast {
 int sweep_variant_count();
 const SweepVariant &sweep_variant(int v);
+// Start levels of the chunk ids act[0..nact) back to 0 (after a dataflow cycle).
+hipError_t launch_starts_reset(hipStream_t st, int *tclean, const int *act, int nact, int ntile);
 hipError_t launch_sweep3d(hipStream_t st, const SweepArgs &a, int variant, int adjoint,
                           int uniform, int D, int s1, int s2, int s3, int ntiles_diag);
 // One dataflow launch (a.active / a.nact: the chunks; f.o0: which half of the cycle;

@@ -193,7 +193,7 @@ __device__ __forceinline__ bool skip_tile(const SweepArgs &a, const TileCtx &t, 
 {
     if (ix < 0) return false;
     auto ld = [](const int *p) { return DEV ? ld_agent(p) : *p; };
-    const int cl = ld(&a.tclean[ix * 8 + a.o]);
+    const int cl = ld(&a.tclean[ix * kTileClean + a.o]);
     if (cl <= 0) return false;
     const int ti = t.i0 / TI, tj = t.j0 / TJ;
     const int *M = a.tmod;
@@ -215,12 +215,12 @@ __device__ __forceinline__ unsigned col_faces(bool valid, int li, int lj, int TI
 
 // Profiling: the node-sweeps this tile sweep performed (columns x depth x the chunk's
 // real sources; padding sources have node -1).
-__device__ __forceinline__ void count_swept(const SweepArgs &a, const TileCtx &t)
+__device__ __forceinline__ void count_swept(const SweepArgs &a, const TileCtx &t, int levels)
 {
     if (!a.swept || threadIdx.x != 0) return;
     int nreal = 0;
     for (int c = 0; c < PFAST_CHUNK; c++) nreal += a.chunks[t.chunk].srcnode[c] >= 0 ? 1 : 0;
-    atomicAdd(a.swept, (unsigned long long)t.TIe * t.TJe * a.nz * nreal);
+    atomicAdd(a.swept, (unsigned long long)t.TIe * t.TJe * levels * nreal);
 }
 
 // Record the outcome of this tile sweep: `what` = bit 4 (some node changed) | the
@@ -241,11 +241,84 @@ __device__ __forceinline__ void record_tile(const SweepArgs &a, int ix, unsigned
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 if ((v >> f) & 1) st_agent(&a.tmod[ix * 5 + 1 + f], a.epoch);
-            st_agent(&a.tclean[ix * 8 + a.o], 0);
+            st_agent(&a.tclean[ix * kTileClean + a.o], 0);
         } else {
-            st_agent(&a.tclean[ix * 8 + a.o], a.epoch);
+            st_agent(&a.tclean[ix * kTileClean + a.o], a.epoch);
         }
     }
+}
+
+// ------------------------------------------------------------ start levels
+// (SweepArgs::starts.)  A block's LDS record array: rec[0] the change word of
+// record_tile, rec[1 + 2r] / rec[2 + 2r] the lowest / highest level (sweep order)
+// changed in region r: 0 the tile, 1..4 its faces x-, x+, y-, y+.
+constexpr int kStartRec = 11;
+
+__device__ __forceinline__ void start_init(int *rec)  // one thread, before a barrier
+{
+    rec[0] = 0;
+    for (int r = 0; r < 5; r++) {
+        rec[1 + 2 * r] = 0x7fffffff;
+        rec[2 + 2 * r] = -1;
+    }
+}
+
+// This lane's changed levels lo..hi (none: hi < lo), into the regions it lies in;
+// before record_tile's barrier.
+__device__ __forceinline__ void start_ranges(int *rec, int lo, int hi, unsigned faces)
+{
+    if (hi < lo) return;
+    atomicMin(&rec[1], lo);
+    atomicMax(&rec[2], hi);
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        if ((faces >> f) & 1u) {
+            atomicMin(&rec[3 + 2 * f], lo);
+            atomicMax(&rec[4 + 2 * f], hi);
+        }
+}
+
+// Record ix of region r of the tile (its own for r = 0, else the neighbour across
+// face r), or -1 outside the grid.
+template <int TI, int TJ>
+__device__ __forceinline__ int region_tile(const SweepArgs &a, const TileCtx &t, int ix, int r)
+{
+    const int ti = t.i0 / TI, tj = t.j0 / TJ;
+    if (r == 1) return tj > 0 ? ix - 1 : -1;
+    if (r == 2) return tj < a.ntj - 1 ? ix + 1 : -1;
+    if (r == 3) return ti > 0 ? ix - a.ntj : -1;
+    if (r == 4) return ti < a.nti - 1 ? ix + a.ntj : -1;
+    return ix;
+}
+
+// After record_tile's barrier, lanes 0..39 (region lane / 8, ordering lane % 8): the
+// start levels this sweep leaves for the tile and the neighbours across its changed
+// faces.  s3: depth direction of this sweep's ordering a.o.
+template <int TI, int TJ>
+__device__ __forceinline__ void start_update(const SweepArgs &a, const TileCtx &t, int ix,
+                                             const int *rec, int s3)
+{
+    const int lane = (int)threadIdx.x;
+    if (lane >= 40) return;
+    const int r = lane >> 3, o2 = lane & 7;
+    const int tgt = region_tile<TI, TJ>(a, t, ix, r);
+    const int lo = rec[1 + 2 * r], hi = rec[2 + 2 * r];
+    const int nz = a.nz;
+    if (r == 0 && o2 == a.o) {  // from this sweep's first change on
+        a.tclean[tgt * kTileClean + 8 + o2] = hi < 0 ? nz : max(lo - 1, 0);
+        return;
+    }
+    if (hi < 0 || tgt < 0) return;
+    // the changed levels in depth, then the first of them in ordering o2's sweep order
+    const int klo = s3 > 0 ? lo : nz - 1 - hi, khi = s3 > 0 ? hi : nz - 1 - lo;
+    const int first = o2 >= 4 ? klo : nz - 1 - khi;
+    atomicMin(&a.tclean[tgt * kTileClean + 8 + o2], r == 0 ? max(first - 1, 0) : first);
+}
+
+// The start level of this block's sweep (0 without start levels).
+__device__ __forceinline__ int start_level(const SweepArgs &a, int ix)
+{
+    return a.starts && ix >= 0 ? a.tclean[ix * kTileClean + 8 + a.o] : 0;
 }
 
 // ------------------------------------------------------------ dataflow waits
@@ -425,10 +498,11 @@ __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepAr
 // chunk's sources hold fixed boxes (half-width R > 0) that the sweep must leave alone;
 // with R = 0 the source node alone is fixed, and no local solve can lower its 0 (each
 // is at least the smallest neighbour value, and traveltimes are never negative), so
-// the sweep needs no test for it.
+// the sweep needs no test for it.  k0: the start level (SweepArgs::starts; 0 in the
+// dataflow kernels); rec: the block's LDS record array (kStartRec ints).
 template <int TI, int TJ, int C, int L, bool UNI, bool FLOW, bool BOX>
 __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int s3, int six, float *pub, int *any_change,
+                                         int s3, int six, int k0, float *pub, int *rec,
                                          FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
@@ -453,7 +527,8 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     const float INF = PFAST_INF_F;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = INF;
-        *any_change = 0;
+        if constexpr (FLOW) rec[0] = 0;
+        else start_init(rec);
     }
     // steps p at which this thread updates its node: plo = pipj <= p < pipj + nz, except
     // the steps pfl .. pfl + pfn at which its column crosses its source's fixed box
@@ -509,17 +584,30 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
         flow_wait(*fw, L - 1);
         asm volatile("" ::: "memory");
     }
-    // window at step 0 (level m0 = -pipj); level m0-1 is outside the column
-    const int m0 = -l.pipj;
-    float vprev = INF;
-    float vcur = l.valid && m0 == 0 ? fload<FLOW>(RF, l.oI) : INF;
-    float vnext = l.valid && m0 >= -1 ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
+    // the sweep starts at step pb0 <= k0 (a multiple of two batches, for the queue and
+    // parity alternation), where the window holds levels m0-1 (as this sweep leaves it:
+    // below k0 that is the stored value), m0 and m0+1 of level m0 = pb0 - pipj
+    // (the dataflow kernels start at step 0: level m0-1 is outside the column)
+    const int pb0 = FLOW ? 0 : k0 / (2 * L) * (2 * L);
+    const int m0 = pb0 - l.pipj;
+    auto lev = [&](int m) { return l.valid && (unsigned)m < (unsigned)nz; };
+    float vprev, vcur, vnext;
+    if constexpr (FLOW) {
+        vprev = INF;
+        vcur = l.valid && m0 == 0 ? fload<FLOW>(RF, l.oI) : INF;
+        vnext = l.valid && m0 >= -1 ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
+    } else {
+        vprev = lev(m0 - 1) ? fload<FLOW>(RF, l.oI + (unsigned)(m0 - 1) * st) : INF;
+        vcur = lev(m0) ? fload<FLOW>(RF, l.oI + (unsigned)m0 * st) : INF;
+        vnext = lev(m0 + 1) ? fload<FLOW>(RF, l.oI + (unsigned)(m0 + 1) * st) : INF;
+    }
     float qa[2][L], qb[2][L], ha[HS::K], hb[HS::K];
-    load_batch(0, qa[0], qa[1], ha);
+    load_batch(pb0, qa[0], qa[1], ha);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();  // INF slot visible
 
     bool changed = false;
+    int mlo = 0x7fffffff, mhi = -1;  // this lane's changed levels, batch granularity
     float sval[L];
     bool chg[L];
 #pragma unroll
@@ -581,11 +669,20 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             vcur = vnext;
             vnext = q[0][u];
         }
+        if constexpr (!FLOW) {
+            bool bc = false;
+#pragma unroll
+            for (int u = 0; u < L; u++) bc = bc || chg[u];
+            if (bc) {
+                mlo = min(mlo, pb - l.pipj);
+                mhi = pb + L - 1 - l.pipj;
+            }
+        }
     };
     // steps 0 .. pend-1, in batches of L (the queues and stage parities alternate); the
     // loop ends after the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
-    for (int pb = 0;; pb += 2 * L) {
+    for (int pb = pb0;; pb += 2 * L) {
         batch(pb, qa, ha, qb, hb);
         if (pb + L >= pend) break;
         batch(pb + L, qb, hb, qa, ha);
@@ -601,26 +698,36 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-    count_swept(a, t);
-    record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
-                any_change);
+    if constexpr (FLOW) {
+        count_swept(a, t, nz);
+        record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
+                    rec);
+    } else {
+        const unsigned faces = col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe);
+        if (a.starts) start_ranges(rec, max(mlo, 0), min(mhi, nz - 1), faces);
+        count_swept(a, t, nz - pb0);
+        record_tile(a, six, changed ? 16u | faces : 0u, rec);
+        if (a.starts && six >= 0) start_update<TI, TJ>(a, t, six, rec, s3);
+    }
 }
 
 template <int TI, int TJ, int C, int L, bool UNI>
 __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s1, int s2, int s3)
 {
     __shared__ float pub[2 * HaloStage<TI, TJ, C, L>::PB];
-    __shared__ int any_change;
+    __shared__ int rec[kStartRec];
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
+    const int k0 = start_level(a, six);
+    if (k0 >= a.nz) return;
     phase_epoch(a);
     if (a.chunks[t.chunk].srcinit > 0)
-        fwd_tile<TI, TJ, C, L, UNI, false, true>(a, t, s1, s2, s3, six, pub, &any_change, nullptr,
+        fwd_tile<TI, TJ, C, L, UNI, false, true>(a, t, s1, s2, s3, six, k0, pub, rec, nullptr,
                                                  nullptr);
     else
-        fwd_tile<TI, TJ, C, L, UNI, false, false>(a, t, s1, s2, s3, six, pub, &any_change, nullptr,
+        fwd_tile<TI, TJ, C, L, UNI, false, false>(a, t, s1, s2, s3, six, k0, pub, rec, nullptr,
                                                   nullptr);
 }
 
@@ -631,7 +738,7 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS, bool FLOW>
 __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int six, float2 *pub, int *any_change, FlowWait *fw,
+                                         int six, int k0, float2 *pub, int *rec, FlowWait *fw,
                                          unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
@@ -659,7 +766,8 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     const float wx = a.wx, wy = a.wy, wz = a.wz;
     if (tid == 0) {
         pub[2 * NT] = pub[PB + 2 * NT] = make_float2(0.0f, PFAST_INF_F);
-        *any_change = 0;
+        if constexpr (FLOW) rec[0] = 0;
+        else start_init(rec);
     }
 
     // the halo faces are loaded by the whole block and published step by step
@@ -679,16 +787,28 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         flow_wait(*fw, L - 1);
         asm volatile("" ::: "memory");
     }
-    const int m0 = -l.pipj;
-    float2 vprev = make_float2(0.0f, 0.0f);
-    float2 vcur = l.valid && m0 == 0 ? ld2(oI) : make_float2(0.0f, 0.0f);
-    float2 vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : make_float2(0.0f, 0.0f);
+    // the window at step pb0 (as in fwd_tile)
+    const int pb0 = FLOW ? 0 : k0 / (2 * L) * (2 * L);
+    const int m0 = pb0 - l.pipj;
+    auto lev = [&](int m) { return l.valid && (unsigned)m < (unsigned)nz; };
+    const float2 zero = make_float2(0.0f, 0.0f);
+    float2 vprev, vcur, vnext;
+    if constexpr (FLOW) {
+        vprev = zero;
+        vcur = l.valid && m0 == 0 ? ld2(oI) : zero;
+        vnext = l.valid && m0 >= -1 ? ld2(oI + (unsigned)(m0 + 1) * st) : zero;
+    } else {
+        vprev = lev(m0 - 1) ? ld2(oI + (unsigned)(m0 - 1) * st) : zero;
+        vcur = lev(m0) ? ld2(oI + (unsigned)m0 * st) : zero;
+        vnext = lev(m0 + 1) ? ld2(oI + (unsigned)(m0 + 1) * st) : zero;
+    }
     float2 qa[1][L], qb[1][L], ha[HS::K], hb[HS::K];
-    load_batch(0, qa[0], ha);
+    load_batch(pb0, qa[0], ha);
     if constexpr (FLOW) flow_prefetch(*fw);
     __syncthreads();
 
     bool changed = false;
+    int mlo = 0x7fffffff, mhi = -1;  // this lane's changed levels, batch granularity
     float sval[L];
     bool chg[L];
 #pragma unroll
@@ -766,11 +886,20 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             vcur = vnext;
             vnext = q[0][u];
         }
+        if constexpr (!FLOW) {
+            bool bc = false;
+#pragma unroll
+            for (int u = 0; u < L; u++) bc = bc || chg[u];
+            if (bc) {
+                mlo = min(mlo, pb - l.pipj);
+                mhi = pb + L - 1 - l.pipj;
+            }
+        }
     };
     // steps 0 .. pend-1, in batches of L (the queues alternate); the loop ends after
     // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
-    for (int pb = 0;; pb += 2 * L) {
+    for (int pb = pb0;; pb += 2 * L) {
         batch(pb, qa, ha, qb, hb);
         if (pb + L >= pend) break;
         batch(pb + L, qb, hb, qa, ha);
@@ -786,23 +915,32 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     if constexpr (FLOW) wait_vm_all();
     const unsigned long long bm = __ballot(changed);
     if ((threadIdx.x & 63) == 0 && bm) atomicOr(&a.changed[t.chunk], chunk_bits(bm, C));
-    count_swept(a, t);
-    record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
-                any_change);
+    if constexpr (FLOW) {
+        count_swept(a, t, nz);
+        record_tile(a, six, changed ? 16u | col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe) : 0u,
+                    rec);
+    } else {
+        const unsigned faces = col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe);
+        if (a.starts) start_ranges(rec, max(mlo, 0), min(mhi, nz - 1), faces);
+        count_swept(a, t, nz - pb0);
+        record_tile(a, six, changed ? 16u | faces : 0u, rec);
+        if (a.starts && six >= 0) start_update<TI, TJ>(a, t, six, rec, s3);
+    }
 }
 
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS>
 __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s1, int s2)
 {
     __shared__ float2 pub[2 * HaloStage<TI, TJ, C, L>::PB];
-    __shared__ int any_change;
+    __shared__ int rec[kStartRec];
     const TileCtx t = decode_tile<TI, TJ>(a, D, s1, s2);
     if (!t.live || !phase_live(a, t.chunk)) return;
     const int six = skip_index<TI, TJ>(a, t);
     if (skip_tile<TI, TJ>(a, t, six)) return;
+    const int k0 = start_level(a, six);
+    if (k0 >= a.nz) return;
     phase_epoch(a);
-    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, pub, &any_change, nullptr,
-                                              nullptr);
+    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, k0, pub, rec, nullptr, nullptr);
 }
 
 // ======================================================= dataflow kernels
@@ -888,7 +1026,7 @@ __device__ __forceinline__ void flow_claim(const SweepArgs &a, const FlowArgs &f
                 s.skip = 2;  // retired chunk: nothing of it runs, nothing waits on it
             } else if (a.tmod) {
                 const int ix = (s.chunk * a.nti + k.ti) * a.ntj + k.tj;
-                const int cl = ld_agent(&a.tclean[ix * 8 + k.o]);
+                const int cl = ld_agent(&a.tclean[ix * kTileClean + k.o]);
                 if (cl > 0 && ld_agent(&a.tmod[ix * 5]) < cl) {
                     // a candidate: the exact test needs the neighbours' final records
                     for (int i = 0; i < n; i++)
@@ -964,7 +1102,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
 {
     flow_phase(f);
     __shared__ float pub[2 * HaloStage<TI, TJ, C, L>::PB];
-    __shared__ int any_change;
+    __shared__ int rec[1];
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
     __syncthreads();
@@ -978,10 +1116,10 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     if (at.chunks[t.chunk].srcinit > 0)
         fwd_tile<TI, TJ, C, L, UNI, true, true>(at, t, pflow::s1_of(o), pflow::s2_of(o),
-                                                S3POS ? 1 : -1, six, pub, &any_change, &fw, self);
+                                                S3POS ? 1 : -1, six, 0, pub, rec, &fw, self);
     else
         fwd_tile<TI, TJ, C, L, UNI, true, false>(at, t, pflow::s1_of(o), pflow::s2_of(o),
-                                                 S3POS ? 1 : -1, six, pub, &any_change, &fw, self);
+                                                 S3POS ? 1 : -1, six, 0, pub, rec, &fw, self);
     flow_finish(f, self);
 }
 
@@ -990,7 +1128,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
 {
     flow_phase(f);
     __shared__ float2 pub[2 * HaloStage<TI, TJ, C, L>::PB];
-    __shared__ int any_change;
+    __shared__ int rec[1];
     __shared__ FlowTaskLds s;
     flow_claim<TI, TJ>(a, f, s);
     __syncthreads();
@@ -1002,8 +1140,8 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     flow_task<TI, TJ>(a, f, s, t, at, fw, self);
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
-    adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, pub,
-                                             &any_change, &fw, self);
+    adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, 0,
+                                             pub, rec, &fw, self);
     flow_finish(f, self);
 }
 
@@ -1415,6 +1553,27 @@ hipError_t launch_flow_reset(hipStream_t st, unsigned *state, size_t words)
 {
     hipLaunchKernelGGL(flow_reset, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, state,
                        (long)words);
+    return hipGetLastError();
+}
+
+namespace {
+__global__ void starts_reset(int *__restrict__ tclean, const int *__restrict__ act, int nact,
+                             int ntile)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // (active slot, tile, ordering)
+    if (e >= (long)nact * ntile * 8) return;
+    const long r = e / 8;
+    const int tile = (int)(r % ntile), slot = (int)(r / ntile);
+    tclean[((long)act[slot] * ntile + tile) * kTileClean + 8 + (int)(e % 8)] = 0;
+}
+}  // namespace
+
+hipError_t launch_starts_reset(hipStream_t st, int *tclean, const int *act, int nact, int ntile)
+{
+    const long n = (long)nact * ntile * 8;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(starts_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tclean,
+                       act, nact, ntile);
     return hipGetLastError();
 }
 

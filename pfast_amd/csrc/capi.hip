@@ -224,10 +224,11 @@ int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams)
     return PFAST_OK;
 }
 
-int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int enable)
+int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int mode)
 {
     ARGCHK(ctx, "ctx is NULL");
-    ctx->tile_skip = enable ? 1 : 0;
+    ARGCHK(mode >= 0 && mode <= 2, "tile skip mode must be 0, 1 or 2");
+    ctx->tile_skip = mode;
     return PFAST_OK;
 }
 
@@ -859,12 +860,12 @@ int run_sweeps3d(pfast_shots3d *s, bool adjoint, int MAXSwp, std::vector<int> &c
             s->ntile = 0;
             int rc;
             if ((rc = pfast::dalloc(ctx, &s->tmod, (size_t)s->nchunk * ntile * 5)) ||
-                (rc = pfast::dalloc(ctx, &s->tclean, (size_t)s->nchunk * ntile * 8)))
+                (rc = pfast::dalloc(ctx, &s->tclean, (size_t)s->nchunk * ntile * kTileClean)))
                 return rc;
             s->ntile = ntile;
         }
         HIPCHK(hipMemsetAsync(s->tmod, 0, (size_t)s->nchunk * ntile * 5 * sizeof(int), st));
-        HIPCHK(hipMemsetAsync(s->tclean, 0, (size_t)s->nchunk * ntile * 8 * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(s->tclean, 0, (size_t)s->nchunk * ntile * kTileClean * sizeof(int), st));
         skip.tmod = s->tmod;
         skip.tclean = s->tclean;
     }
@@ -975,6 +976,7 @@ SweepArgs sweep_args(const pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunk
     const bool skipping = skip && sv.kind == 0;
     a.tmod = skipping ? skip->tmod : nullptr;
     a.tclean = skipping ? skip->tclean : nullptr;
+    a.starts = skipping && ctx->tile_skip == 2 ? 1 : 0;
     a.ntA = 0x7fffffff;
     return a;
 }
@@ -1034,6 +1036,9 @@ int issue_cycle3d_flow(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, c
         HIPCHK(launch_sweep3d_flow(st, a, f, ctx->sweep, adjoint ? 1 : 0, g.uniform));
         time_end(ctx, st);
     }
+    // the dataflow kernels do not keep the start levels: sweep these chunks' tiles from
+    // their first level in later per-diagonal launches
+    if (a.starts) HIPCHK(launch_starts_reset(st, a.tclean, act, nact, ntile));
     return PFAST_OK;
 }
 

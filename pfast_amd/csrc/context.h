@@ -44,7 +44,9 @@ struct pfast_ctx {
     // batched 3D cycles: the active chunks split over `nstreams` streams (stream and
     // aux[0..nstreams-2]) so that one stream's launch tails fill with another's work
     int nstreams = 1;
-    int tile_skip = 1;  // exact tile skipping in the column-window sweeps
+    // exact tile skipping in the column-window sweeps: 0 off, 1 whole tiles, 2 whole
+    // tiles and start levels (SweepArgs::starts)
+    int tile_skip = 2;
     int dataflow = 2;   // batched 3D cycles as dataflow launches: 0 never, 1 always, 2 auto
     int paired = 1;     // per-diagonal launches pair orderings 2p and 2p+1 (plan.h)
     // batched 3D phases: sweep cycles issued between two convergence checks on the host

@@ -187,9 +187,9 @@ int grow_group(Group &G, int need)
     int *tm = nullptr, *tc = nullptr;
     if (G.ntile &&
         (hipMalloc(&tm, (size_t)cap * G.ntile * 5 * sizeof(int)) != hipSuccess ||
-         hipMalloc(&tc, (size_t)cap * G.ntile * 8 * sizeof(int)) != hipSuccess ||
+         hipMalloc(&tc, (size_t)cap * G.ntile * kTileClean * sizeof(int)) != hipSuccess ||
          hipMemsetAsync(tm, 0, (size_t)cap * G.ntile * 5 * sizeof(int), G.ctx->stream) != hipSuccess ||
-         hipMemsetAsync(tc, 0, (size_t)cap * G.ntile * 8 * sizeof(int), G.ctx->stream) != hipSuccess)) {
+         hipMemsetAsync(tc, 0, (size_t)cap * G.ntile * kTileClean * sizeof(int), G.ctx->stream) != hipSuccess)) {
         (void)hipGetLastError();
         for (void *q : {(void *)dt, (void *)da, (void *)dc, (void *)tm, (void *)tc})
             if (q) (void)hipFree(q);
@@ -443,13 +443,13 @@ int issue_cycle(pfast_server *S, Group &G)
         const long nd = pfast::diagonals3d(G.ctx, G.g);
         if ((G.cycle_no + 2) * 8 * nd >= 0x7fffffffL) {
             HIPCHK(hipMemsetAsync(G.tmod, 0, (size_t)G.cap * G.ntile * 5 * sizeof(int), st));
-            HIPCHK(hipMemsetAsync(G.tclean, 0, (size_t)G.cap * G.ntile * 8 * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tclean, 0, (size_t)G.cap * G.ntile * kTileClean * sizeof(int), st));
             G.cycle_no = 0;
             G.reset.clear();
         }
         for (int id : G.reset) {
             HIPCHK(hipMemsetAsync(G.tmod + (size_t)id * G.ntile * 5, 0, G.ntile * 5 * sizeof(int), st));
-            HIPCHK(hipMemsetAsync(G.tclean + (size_t)id * G.ntile * 8, 0, G.ntile * 8 * sizeof(int), st));
+            HIPCHK(hipMemsetAsync(G.tclean + (size_t)id * G.ntile * kTileClean, 0, G.ntile * kTileClean * sizeof(int), st));
         }
         G.reset.clear();
     }

@@ -167,40 +167,80 @@ def test_stream_split_bitwise_identical():
     (46, (16, 25, 41), (12.5, 10.0, 8.0)),
 ])
 def test_tile_skip_bitwise_identical(seed, shape, spacing):
-    """Exact tile skipping only leaves out sweeps that would change nothing: fields,
-    cycle counts, gradient and misfit equal those with every tile swept."""
+    """Exact tile skipping only leaves out sweeps, and start levels only the levels of a
+    sweep, that would change nothing: fields, cycle counts, gradient and misfit equal
+    those with every tile swept, in the per-diagonal launches (where start levels
+    apply) and in the dataflow launches."""
     import pfast_amd
     c = cases.case3d(seed, shape=shape, spacing=spacing, nsrc=9, nrec_per=12)
-    outs = []
-    for skip in (False, True):
+    outs = {}
+    for flow, mode in ((0, 0), (0, 1), (0, 2), (1, 2)):
         ctx = pfast_amd.Context(0)
-        ctx.set_tile_skip(skip)
+        ctx.set_dataflow(flow)
+        ctx.set_tile_skip(mode)
         m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
         sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
         sh.run(adjoint=True, MAXSwp=100)
         g, E = sh.gradient()
-        outs.append((np.stack([sh.field(s) for s in range(9)]),
-                     np.stack([sh.field(s, adjoint=True) for s in range(9)]), sh.cycles(), g, E))
+        outs[flow, mode] = (np.stack([sh.field(s) for s in range(9)]),
+                            np.stack([sh.field(s, adjoint=True) for s in range(9)]), sh.cycles(),
+                            g, E)
         sh.close()
         m.close()
         ctx.close()
-    a, b = outs
-    print(f"cycles {a[2].tolist()}", flush=True)
-    assert all(np.array_equal(cases.bits(x), cases.bits(y)) for x, y in zip(a[:4], b[:4]))
-    assert a[4] == b[4]
+    ref = outs[0, 0]
+    print(f"cycles {ref[2].tolist()}", flush=True)
+    for key, out in outs.items():
+        assert all(np.array_equal(cases.bits(x), cases.bits(y))
+                   for x, y in zip(out[:4], ref[:4])), key
+        assert out[4] == ref[4], key
+
+
+def test_start_levels_across_schedule_switch():
+    """A phase that starts in per-diagonal launches and switches to dataflow launches as
+    chunks converge (auto schedule: 12 chunks x 41 x 41 tiles of 4 x 4 columns is just
+    above the switch, 11 below it), and back in the next phase: the start levels kept by
+    the per-diagonal launches are reset after each dataflow cycle, so every mode gives
+    the same bits."""
+    import pfast_amd
+    names = pfast_amd.sweep_variants()
+    v44 = [i for i, n in enumerate(names) if " 4x4 " in n][0]
+    c = cases.case3d(61, shape=(12, 161, 161), spacing=(25.0, 25.0, 25.0), nsrc=96,
+                     nrec_per=10)
+    outs = {}
+    for mode in (0, 1, 2):
+        ctx = pfast_amd.Context(0)
+        ctx.set_sweep(v44)
+        ctx.set_tile_skip(mode)
+        m = pfast_amd.Model3D(ctx, c["vp"], c["dx"], c["dy"], c["dz"])
+        sh = pfast_amd.Shots3D(m, c["src"], c["srnum"], c["rec"], c["nxyz"], c["T_obs"])
+        sh.run(adjoint=True, MAXSwp=100)
+        g, E = sh.gradient()
+        outs[mode] = (np.stack([sh.field(s) for s in range(0, 96, 7)]),
+                      np.stack([sh.field(s, adjoint=True) for s in range(0, 96, 7)]),
+                      sh.cycles(), g, E)
+        sh.close()
+        m.close()
+        ctx.close()
+    print(f"cycles {np.unique(outs[0][2], axis=0).tolist()}", flush=True)
+    for mode in (1, 2):
+        assert all(np.array_equal(cases.bits(x), cases.bits(y))
+                   for x, y in zip(outs[mode][:4], outs[0][:4])), mode
+        assert outs[mode][4] == outs[0][4], mode
 
 
 @pytest.mark.parametrize("variant", [0, 5, 6, 8])
 def test_paired_launches_bitwise_identical(variant):
     """Per-diagonal launches that pair orderings 2p and 2p+1 (the second trailing by
     ntj + 1 tile diagonals) sweep every tile with the values the one-ordering launches
-    give it: fields, cycle counts, gradient and misfit are the same bits, with and
-    without tile skipping, for tile shapes with different ntj."""
+    give it: fields, cycle counts, gradient and misfit are the same bits, without tile
+    skipping, with whole tiles skipped and with start levels, for tile shapes with
+    different ntj."""
     import pfast_amd
     c = cases.case3d(47 + variant, shape=(19, 35, 41), spacing=(25.0, 25.0, 25.0), nsrc=11,
                      nrec_per=12)
     outs = {}
-    for skip in (False, True):
+    for skip in (0, 1, 2):
         for paired in (False, True):
             ctx = pfast_amd.Context(0)
             ctx.set_sweep(variant)
