@@ -332,9 +332,9 @@ def main():
                     help="per-diagonal launches pair orderings 2p and 2p+1, the second "
                          "trailing by ntj + 1 tile diagonals (same results)")
     ap.add_argument("--tile-skip", choices=["off", "tiles", "levels"], default="levels",
-                    help="exact tile skipping: off, whole tiles, or whole tiles and start "
-                         "levels (a sweep starts at the first level whose inputs changed); "
-                         "same results")
+                    help="exact tile skipping: off, whole tiles, or whole tiles and depth "
+                         "ranges (a sweep starts at the first and ends after the last level "
+                         "whose inputs changed); same results")
     ap.add_argument("--no-tile-skip", action="store_true", help="same as --tile-skip off")
     ap.add_argument("--streams", type=int, default=3,
                     help="streams a batched cycle splits its source chunks over in the timed "

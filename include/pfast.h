@@ -223,9 +223,10 @@ int pfast_ctx_set_streams(pfast_ctx *ctx, int nstreams);
 /* Exact tile skipping of the batched 3D sweeps: 0 = off; 1 = whole tiles: a tile
  * whose last sweep in an ordering changed nothing, and whose nodes and neighbour
  * tiles have not changed since, is not swept again in that ordering; 2 (default) =
- * whole tiles and start levels: the per-diagonal launches also start a tile's sweep
+ * whole tiles and depth ranges: the per-diagonal launches also start a tile's sweep
  * at the first level whose inputs differ from what the ordering's last sweep of the
- * tile saw (the levels above it would be reproduced).  Results are the same. */
+ * tile saw, and end it once it is past the last such level and a batch of steps
+ * changed nothing (the levels outside would be reproduced).  Results are the same. */
 int pfast_ctx_set_tile_skip(pfast_ctx *ctx, int mode);
 /* Paired per-diagonal launches of the batched 3D sweeps (on by default): orderings 2p
  * and 2p+1 of a cycle, which differ only in the column direction (the innermost loop of

@@ -173,7 +173,8 @@ class Context:
 
     def set_tile_skip(self, mode=True):
         """Exact tile skipping of the batched 3D sweeps: False/0 off, 1 whole tiles,
-        True/2 (default) whole tiles and start levels (same results)."""
+        True/2 (default) whole tiles and depth ranges (a sweep starts at the first and
+        ends after the last level whose inputs changed; same results)."""
         m = 2 if mode is True else 0 if mode is False else int(mode)
         check(lib().pfast_ctx_set_tile_skip(self.h, m))
 

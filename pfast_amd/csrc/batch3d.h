@@ -21,7 +21,8 @@ struct ChunkRef {
                                // the source node alone, as the reference seeds it)
 };
 
-constexpr int kTileClean = 16;  // tclean ints per (chunk, tile): 8 epochs, 8 start levels
+// tclean ints per (chunk, tile): 8 clean epochs, 8 start levels, 8 end levels
+constexpr int kTileClean = 24;
 
 struct SweepArgs {
     const ChunkRef *chunks;  // indexed by chunk id
@@ -49,9 +50,13 @@ struct SweepArgs {
     // it to its own first changed level less one (a node's update reads the levels
     // beside it), and every change lowers it for the other orderings of the tile (by
     // the same margin) and, on a face, for all orderings of the neighbour across it
-    // (to the changed level: a halo node is read at its own level only).  The
-    // dataflow kernels do not keep them: a dataflow cycle resets its chunks' start
-    // levels to 0 afterwards (launch_starts_reset).  kTileClean ints per (chunk, tile).
+    // (to the changed level: a halo node is read at its own level only).
+    // End levels, likewise: tclean[16 + o] = the number of trailing levels (sweep
+    // order) whose inputs are the ones o's last sweep saw, so that 0 means none (as
+    // the records start).  Beyond the last dirty level a sweep reproduces the stored
+    // values once a whole batch of steps has changed nothing, and ends there.  The
+    // dataflow kernels do not keep them: a dataflow cycle resets its chunks' start and
+    // end levels to 0 afterwards (launch_starts_reset).
     int *tmod, *tclean;
     int epoch, o;         // this launch's epoch (increasing within a phase), ordering
     // A paired launch (plan.h) sweeps two orderings that differ in the column direction

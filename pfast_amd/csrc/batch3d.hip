@@ -251,8 +251,9 @@ __device__ __forceinline__ void record_tile(const SweepArgs &a, int ix, unsigned
 // ------------------------------------------------------------ start levels
 // (SweepArgs::starts.)  A block's LDS record array: rec[0] the change word of
 // record_tile, rec[1 + 2r] / rec[2 + 2r] the lowest / highest level (sweep order)
-// changed in region r: 0 the tile, 1..4 its faces x-, x+, y-, y+.
-constexpr int kStartRec = 11;
+// changed in region r: 0 the tile, 1..4 its faces x-, x+, y-, y+; rec[11 + b % 3]
+// whether batch b changed a node (kStartRec ints).
+constexpr int kStartRec = 14;
 
 __device__ __forceinline__ void start_init(int *rec)  // one thread, before a barrier
 {
@@ -261,6 +262,7 @@ __device__ __forceinline__ void start_init(int *rec)  // one thread, before a ba
         rec[1 + 2 * r] = 0x7fffffff;
         rec[2 + 2 * r] = -1;
     }
+    rec[11] = rec[12] = rec[13] = 0;
 }
 
 // This lane's changed levels lo..hi (none: hi < lo), into the regions it lies in;
@@ -304,21 +306,31 @@ __device__ __forceinline__ void start_update(const SweepArgs &a, const TileCtx &
     const int tgt = region_tile<TI, TJ>(a, t, ix, r);
     const int lo = rec[1 + 2 * r], hi = rec[2 + 2 * r];
     const int nz = a.nz;
-    if (r == 0 && o2 == a.o) {  // from this sweep's first change on
-        a.tclean[tgt * kTileClean + 8 + o2] = hi < 0 ? nz : max(lo - 1, 0);
+    int *w = a.tclean + (size_t)max(tgt, 0) * kTileClean;
+    if (r == 0 && o2 == a.o) {  // this sweep's changed levels and the ones beside them
+        w[8 + o2] = hi < 0 ? nz : max(lo - 1, 0);
+        w[16 + o2] = hi < 0 ? nz : nz - 1 - min(hi + 1, nz - 1);
         return;
     }
     if (hi < 0 || tgt < 0) return;
-    // the changed levels in depth, then the first of them in ordering o2's sweep order
+    // the changed levels in depth, then in ordering o2's sweep order (the tile's own
+    // nodes widened by the level beside them)
     const int klo = s3 > 0 ? lo : nz - 1 - hi, khi = s3 > 0 ? hi : nz - 1 - lo;
-    const int first = o2 >= 4 ? klo : nz - 1 - khi;
-    atomicMin(&a.tclean[tgt * kTileClean + 8 + o2], r == 0 ? max(first - 1, 0) : first);
+    const int first = o2 >= 4 ? klo : nz - 1 - khi, last = o2 >= 4 ? khi : nz - 1 - klo;
+    const int m = r == 0 ? 1 : 0;
+    atomicMin(&w[8 + o2], max(first - m, 0));
+    atomicMin(&w[16 + o2], nz - 1 - min(last + m, nz - 1));
 }
 
-// The start level of this block's sweep (0 without start levels).
+// The start level of this block's sweep (0 without start levels) and its last dirty
+// level (nz - 1 without).
 __device__ __forceinline__ int start_level(const SweepArgs &a, int ix)
 {
     return a.starts && ix >= 0 ? a.tclean[ix * kTileClean + 8 + a.o] : 0;
+}
+__device__ __forceinline__ int last_dirty_level(const SweepArgs &a, int ix)
+{
+    return a.nz - 1 - (a.starts && ix >= 0 ? a.tclean[ix * kTileClean + 16 + a.o] : 0);
 }
 
 // ------------------------------------------------------------ dataflow waits
@@ -498,11 +510,12 @@ __device__ __forceinline__ HaloStage<TI, TJ, C, L> make_halo_stage(const SweepAr
 // chunk's sources hold fixed boxes (half-width R > 0) that the sweep must leave alone;
 // with R = 0 the source node alone is fixed, and no local solve can lower its 0 (each
 // is at least the smallest neighbour value, and traveltimes are never negative), so
-// the sweep needs no test for it.  k0: the start level (SweepArgs::starts; 0 in the
-// dataflow kernels); rec: the block's LDS record array (kStartRec ints).
+// the sweep needs no test for it.  k0, k1: the start level and the last dirty level
+// (SweepArgs::starts; 0 and nz - 1 in the dataflow kernels); rec: the block's LDS
+// record array (kStartRec ints).
 template <int TI, int TJ, int C, int L, bool UNI, bool FLOW, bool BOX>
 __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int s3, int six, int k0, float *pub, int *rec,
+                                         int s3, int six, int k0, int k1, float *pub, int *rec,
                                          FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
@@ -617,7 +630,14 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     }
     // one batch of L steps from queue q and halo values h (loaded a batch earlier);
     // issues the loads of the following batch into nq, nh first
-    auto batch = [&](int pb, float (*q)[L], const float *h, float (*nq)[L], float *nh) {
+    // (per-diagonal kernels) steps from pstop on reproduce the stored values once a whole
+    // batch has changed nothing: every lane is then past the last dirty level and the
+    // level beside it
+    const int pstop = k1 + 2 + t.span;
+    auto batch = [&](int pb, float (*q)[L], const float *h, float (*nq)[L], float *nh) -> bool {
+        const int b = (pb - pb0) / L;
+        if constexpr (!FLOW)
+            if (tid == 0) rec[11 + (b + 1) % 3] = 0;
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
@@ -645,6 +665,8 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
             for (int k = 0; k < HS::K; k++)
                 if (hs.uk[k] == u) P[2 * NT + 1 + hs.rk[k]] = h[k];
             lds_barrier();
+            if constexpr (!FLOW)
+                if (u == 0 && b > 0 && pb >= pstop && rec[11 + (b - 1) % 3] == 0) return true;
             if constexpr (FLOW) {
                 // every wave has waited for its stores of steps < pb - L
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
@@ -677,15 +699,24 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
                 mlo = min(mlo, pb - l.pipj);
                 mhi = pb + L - 1 - l.pipj;
             }
+            if (__ballot(bc) && (tid & 63) == 0) atomicOr(&rec[11 + b % 3], 1);
         }
+        return false;
     };
     // steps 0 .. pend-1, in batches of L (the queues and stage parities alternate); the
     // loop ends after the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
+    int pdone = pend;  // the step the sweep ended at
     for (int pb = pb0;; pb += 2 * L) {
-        batch(pb, qa, ha, qb, hb);
+        if (batch(pb, qa, ha, qb, hb)) {
+            pdone = pb;
+            break;
+        }
         if (pb + L >= pend) break;
-        batch(pb + L, qb, hb, qa, ha);
+        if (batch(pb + L, qb, hb, qa, ha)) {
+            pdone = pb + L;
+            break;
+        }
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -705,7 +736,7 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
     } else {
         const unsigned faces = col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe);
         if (a.starts) start_ranges(rec, max(mlo, 0), min(mhi, nz - 1), faces);
-        count_swept(a, t, nz - pb0);
+        count_swept(a, t, max(min(pdone, nz) - pb0, 0));
         record_tile(a, six, changed ? 16u | faces : 0u, rec);
         if (a.starts && six >= 0) start_update<TI, TJ>(a, t, six, rec, s3);
     }
@@ -722,13 +753,14 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
     if (skip_tile<TI, TJ>(a, t, six)) return;
     const int k0 = start_level(a, six);
     if (k0 >= a.nz) return;
+    const int k1 = last_dirty_level(a, six);
     phase_epoch(a);
     if (a.chunks[t.chunk].srcinit > 0)
-        fwd_tile<TI, TJ, C, L, UNI, false, true>(a, t, s1, s2, s3, six, k0, pub, rec, nullptr,
-                                                 nullptr);
+        fwd_tile<TI, TJ, C, L, UNI, false, true>(a, t, s1, s2, s3, six, k0, k1, pub, rec,
+                                                 nullptr, nullptr);
     else
-        fwd_tile<TI, TJ, C, L, UNI, false, false>(a, t, s1, s2, s3, six, k0, pub, rec, nullptr,
-                                                  nullptr);
+        fwd_tile<TI, TJ, C, L, UNI, false, false>(a, t, s1, s2, s3, six, k0, k1, pub, rec,
+                                                  nullptr, nullptr);
 }
 
 // Adjoint, column window, on the interleaved (lambda, T) field: every load brings a
@@ -738,8 +770,8 @@ __global__ __launch_bounds__(TI *TJ *C) void fwd3d_col(SweepArgs a, int D, int s
 // every neighbour they read is inside the domain.
 template <int TI, int TJ, int C, int L, bool UNI, bool S3POS, bool FLOW>
 __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, int s1, int s2,
-                                         int six, int k0, float2 *pub, int *rec, FlowWait *fw,
-                                         unsigned *self)
+                                         int six, int k0, int k1, float2 *pub, int *rec,
+                                         FlowWait *fw, unsigned *self)
 {
     constexpr int NT = TI * TJ * C;
     constexpr int s3 = S3POS ? 1 : -1;
@@ -816,7 +848,11 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         sval[u] = 0.0f;
         chg[u] = false;
     }
-    auto batch = [&](int pb, float2 (*q)[L], const float2 *h, float2 (*nq)[L], float2 *nh) {
+    const int pstop = k1 + 2 + t.span;  // as in fwd_tile
+    auto batch = [&](int pb, float2 (*q)[L], const float2 *h, float2 (*nq)[L], float2 *nh) -> bool {
+        const int b = (pb - pb0) / L;
+        if constexpr (!FLOW)
+            if (tid == 0) rec[11 + (b + 1) % 3] = 0;
         wait_vm_all();
         if constexpr (FLOW) {
             flow_wait(*fw, pb + 2 * L - 1);
@@ -844,6 +880,8 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
             for (int k = 0; k < HS::K; k++)
                 if (hs.uk[k] == u) P[2 * NT + 1 + hs.rk[k]] = h[k];
             lds_barrier();
+            if constexpr (!FLOW)
+                if (u == 0 && b > 0 && pb >= pstop && rec[11 + (b - 1) % 3] == 0) return true;
             if constexpr (FLOW) {
                 if (u == 0 && tid == 0 && pbo > L) st_agent(self, fw->base + (unsigned)(pbo - L));
             }
@@ -894,15 +932,24 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
                 mlo = min(mlo, pb - l.pipj);
                 mhi = pb + L - 1 - l.pipj;
             }
+            if (__ballot(bc) && (tid & 63) == 0) atomicOr(&rec[11 + b % 3], 1);
         }
+        return false;
     };
     // steps 0 .. pend-1, in batches of L (the queues alternate); the loop ends after
     // the batch holding step pend-1 (uniform over the block)
     const int pend = nz + t.span;
+    int pdone = pend;  // the step the sweep ended at
     for (int pb = pb0;; pb += 2 * L) {
-        batch(pb, qa, ha, qb, hb);
+        if (batch(pb, qa, ha, qb, hb)) {
+            pdone = pb;
+            break;
+        }
         if (pb + L >= pend) break;
-        batch(pb + L, qb, hb, qa, ha);
+        if (batch(pb + L, qb, hb, qa, ha)) {
+            pdone = pb + L;
+            break;
+        }
         if (pb + 2 * L >= pend) break;
     }
     {  // changed nodes of the last batch, the one holding step pend-1
@@ -922,7 +969,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
     } else {
         const unsigned faces = col_faces(l.valid, l.li, l.lj, t.TIe, t.TJe);
         if (a.starts) start_ranges(rec, max(mlo, 0), min(mhi, nz - 1), faces);
-        count_swept(a, t, nz - pb0);
+        count_swept(a, t, max(min(pdone, nz) - pb0, 0));
         record_tile(a, six, changed ? 16u | faces : 0u, rec);
         if (a.starts && six >= 0) start_update<TI, TJ>(a, t, six, rec, s3);
     }
@@ -939,8 +986,10 @@ __global__ __launch_bounds__(TI *TJ *C) void adj3d_col(SweepArgs a, int D, int s
     if (skip_tile<TI, TJ>(a, t, six)) return;
     const int k0 = start_level(a, six);
     if (k0 >= a.nz) return;
+    const int k1 = last_dirty_level(a, six);
     phase_epoch(a);
-    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, k0, pub, rec, nullptr, nullptr);
+    adj_tile<TI, TJ, C, L, UNI, S3POS, false>(a, t, s1, s2, six, k0, k1, pub, rec, nullptr,
+                                              nullptr);
 }
 
 // ======================================================= dataflow kernels
@@ -1116,10 +1165,12 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     if (at.chunks[t.chunk].srcinit > 0)
         fwd_tile<TI, TJ, C, L, UNI, true, true>(at, t, pflow::s1_of(o), pflow::s2_of(o),
-                                                S3POS ? 1 : -1, six, 0, pub, rec, &fw, self);
+                                                S3POS ? 1 : -1, six, 0, a.nz - 1, pub, rec, &fw,
+                                                self);
     else
         fwd_tile<TI, TJ, C, L, UNI, true, false>(at, t, pflow::s1_of(o), pflow::s2_of(o),
-                                                 S3POS ? 1 : -1, six, 0, pub, rec, &fw, self);
+                                                 S3POS ? 1 : -1, six, 0, a.nz - 1, pub, rec, &fw,
+                                                 self);
     flow_finish(f, self);
 }
 
@@ -1141,7 +1192,7 @@ __global__ __launch_bounds__(TI *TJ *C) __attribute__((amdgpu_waves_per_eu(8))) 
     const int o = at.o;
     const int six = a.tmod ? (t.chunk * a.nti + t.i0 / TI) * a.ntj + t.j0 / TJ : -1;
     adj_tile<TI, TJ, C, L, UNI, S3POS, true>(at, t, pflow::s1_of(o), pflow::s2_of(o), six, 0,
-                                             pub, rec, &fw, self);
+                                             a.nz - 1, pub, rec, &fw, self);
     flow_finish(f, self);
 }
 
@@ -1560,17 +1611,18 @@ namespace {
 __global__ void starts_reset(int *__restrict__ tclean, const int *__restrict__ act, int nact,
                              int ntile)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // (active slot, tile, ordering)
-    if (e >= (long)nact * ntile * 8) return;
-    const long r = e / 8;
+    // (active slot, tile, start or end level of an ordering)
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nact * ntile * 16) return;
+    const long r = e / 16;
     const int tile = (int)(r % ntile), slot = (int)(r / ntile);
-    tclean[((long)act[slot] * ntile + tile) * kTileClean + 8 + (int)(e % 8)] = 0;
+    tclean[((long)act[slot] * ntile + tile) * kTileClean + 8 + (int)(e % 16)] = 0;
 }
 }  // namespace
 
 hipError_t launch_starts_reset(hipStream_t st, int *tclean, const int *act, int nact, int ntile)
 {
-    const long n = (long)nact * ntile * 8;
+    const long n = (long)nact * ntile * 16;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(starts_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tclean,
                        act, nact, ntile);

@@ -167,9 +167,9 @@ def test_stream_split_bitwise_identical():
     (46, (16, 25, 41), (12.5, 10.0, 8.0)),
 ])
 def test_tile_skip_bitwise_identical(seed, shape, spacing):
-    """Exact tile skipping only leaves out sweeps, and start levels only the levels of a
+    """Exact tile skipping only leaves out sweeps, and depth ranges only the levels of a
     sweep, that would change nothing: fields, cycle counts, gradient and misfit equal
-    those with every tile swept, in the per-diagonal launches (where start levels
+    those with every tile swept, in the per-diagonal launches (where depth ranges
     apply) and in the dataflow launches."""
     import pfast_amd
     c = cases.case3d(seed, shape=shape, spacing=spacing, nsrc=9, nrec_per=12)
@@ -199,7 +199,7 @@ def test_tile_skip_bitwise_identical(seed, shape, spacing):
 def test_start_levels_across_schedule_switch():
     """A phase that starts in per-diagonal launches and switches to dataflow launches as
     chunks converge (auto schedule: 12 chunks x 41 x 41 tiles of 4 x 4 columns is just
-    above the switch, 11 below it), and back in the next phase: the start levels kept by
+    above the switch, 11 below it), and back in the next phase: the depth ranges kept by
     the per-diagonal launches are reset after each dataflow cycle, so every mode gives
     the same bits."""
     import pfast_amd
@@ -234,7 +234,7 @@ def test_paired_launches_bitwise_identical(variant):
     """Per-diagonal launches that pair orderings 2p and 2p+1 (the second trailing by
     ntj + 1 tile diagonals) sweep every tile with the values the one-ordering launches
     give it: fields, cycle counts, gradient and misfit are the same bits, without tile
-    skipping, with whole tiles skipped and with start levels, for tile shapes with
+    skipping, with whole tiles skipped and with depth ranges, for tile shapes with
     different ntj."""
     import pfast_amd
     c = cases.case3d(47 + variant, shape=(19, 35, 41), spacing=(25.0, 25.0, 25.0), nsrc=11,
