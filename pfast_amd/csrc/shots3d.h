@@ -116,9 +116,10 @@ int issue_steps3d(pfast_ctx *ctx, const Grid3 &g, const ChunkRef *chunks, const 
                   const int *dphase = nullptr);
 // Active chunks x tiles of one ordering up to which the automatic schedule takes the
 // dataflow launches: they keep the GPU busy where a launch per tile anti-diagonal
-// has too few tiles (measured on 201x201x101: 2.0x faster with 64 sources, 1.4x with
-// 128, about equal near 20000, and 0.8x with 400 sources, whose diagonals fill the GPU).
-constexpr long kAutoDataflowTasks = 20000;
+// has too few tiles, while the per-diagonal launches sweep only the depth ranges that
+// can change (measured on 201x201x101, 676 tiles: dataflow 1.7x faster with 32 sources,
+// 1.5x with 64, 1.1x with 128 (10816 tasks), 0.88x with 200 (16900)).
+constexpr long kAutoDataflowTasks = 13000;
 // Whether a cycle over nact chunks of a grid of ntile tiles runs as dataflow launches
 // under ctx's schedule setting (0 never, 1 always, 2 automatic); words: the size of
 // the dataflow state.

@@ -198,8 +198,8 @@ def test_tile_skip_bitwise_identical(seed, shape, spacing):
 
 def test_start_levels_across_schedule_switch():
     """A phase that starts in per-diagonal launches and switches to dataflow launches as
-    chunks converge (auto schedule: 12 chunks x 41 x 41 tiles of 4 x 4 columns is just
-    above the switch, 11 below it), and back in the next phase: the depth ranges kept by
+    chunks converge (auto schedule: 12 chunks x 41 x 41 tiles of 4 x 4 columns are above
+    the switch, 7 below it), and back in the next phase: the depth ranges kept by
     the per-diagonal launches are reset after each dataflow cycle, so every mode gives
     the same bits."""
     import pfast_amd
