@@ -884,6 +884,11 @@ int pfast_server_submit3d(pfast_server *S, int model_id, int kind, int nsrc, con
         R.tobs.assign(T_obs, T_obs + nrec);
     }
     const pfast_model3d *m = it->second.m3;
+    // the sweep kernels address one chunk's field with 32-bit byte offsets, as for a shot set
+    const long plane = (long)m->nrows * m->ncols, N = plane * m->nz;
+    ARGCHK((N + 64L * plane) * PFAST_CHUNK * (kind == PFAST_REQ_GRADIENT ? 8 : 4) <= (1L << 32),
+           "grid too large for the batched solver (one source chunk's field plus 64 levels "
+           "must stay within 4 GiB)");
     R.bytes = footprint3d(pfast::grid_of(m), nsrc, nrec, kind == PFAST_REQ_GRADIENT);
     return enqueue(S, std::move(R), ticket);
 }

@@ -172,6 +172,24 @@ def test_same_result_alone_and_in_mix():
     ctx.close()
 
 
+def test_chunk_field_bound():
+    """The sweep kernels address a chunk's field with 32-bit byte offsets, levels up to 64
+    past either end of a column included, as for a shot set: on a 1700 x 200 x 200 model a
+    gradient request (the (lambda, T) field: 4.52 GB per chunk with the margin) is refused
+    at submission, before anything is queued; the model stays usable."""
+    import pfast_amd
+    nz, nr, nc = 1700, 200, 200
+    srv = pfast_amd.Server(0, 16 << 30)
+    mid = srv.add_model3d(np.full((nz, nr, nc), 2000.0), 25.0, 25.0, 25.0)
+    src = np.array([[nr // 2, nc // 2, 0]])
+    with pytest.raises(pfast_amd.PfastError, match="too large"):
+        srv.submit3d(mid, "gradient", src, np.array([1]), np.array([[nr // 2, nc // 2 + 4, 0]]),
+                     np.array([[0.0, 0.0, 1.0]]), np.array([0.1]))
+    assert srv.info()["queued"] == 0 and srv.info()["running"] == 0
+    srv.remove_model(mid)
+    srv.close()
+
+
 def test_admission_by_footprint():
     import pfast_amd
     models = _models()[:1]
