@@ -128,7 +128,10 @@ int pfast_model3d_destroy(pfast_model3d *m);
  * reference's srcid order), srnum per-source receiver counts, and the flattened
  * receiver table (rec: 3*nrec node indices, nxyz: 3*nrec normals (n_y,n_x,n_z),
  * T_obs: nrec picks; a pick is used when T_obs < 100, as in the reference).
- * T_obs/nxyz/rec may be NULL for forward-only use. */
+ * T_obs/nxyz/rec may be NULL for forward-only use.  The sweep kernels address one
+ * chunk's field (8 sources) with 32-bit byte offsets: a grid whose (N + 64 planes) x 32
+ * bytes (x 64 with picks, the adjoint's (lambda, T) pairs) pass 4 GiB is refused
+ * (PFAST_ERR_ARG). */
 int pfast_shots3d_create(pfast_model3d *m, int nsrc, const int *src, const int *srnum,
                          const int *rec, const double *nxyz, const double *T_obs,
                          pfast_shots3d **out);
@@ -391,7 +394,9 @@ int pfast_server_add_model2d(pfast_server *srv, const double *vp, int nrows, int
 int pfast_server_remove_model(pfast_server *srv, int model_id);
 /* Arrays as for pfast_shots3d_create / pfast_shots2d_create, copied at submission.
  * A table request needs no receivers; a request larger than the budget is refused
- * (PFAST_ERR_NOMEM).  ticket receives the request id. */
+ * (PFAST_ERR_NOMEM), and a 3D request on a grid past pfast_shots3d_create's 4 GiB chunk
+ * field bound (a gradient request by its (lambda, T) field) with PFAST_ERR_ARG.  ticket
+ * receives the request id. */
 int pfast_server_submit3d(pfast_server *srv, int model_id, int kind, int nsrc, const int *src,
                           const int *srnum, const int *rec, const double *nxyz,
                           const double *T_obs, int MAXSwp, long *ticket);
