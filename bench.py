@@ -534,6 +534,14 @@ def main():
         "fp32_cycles_mean": [round(float(cyc[:, 0].mean()), 3), round(float(cyc[:, 1].mean()), 3)],
         "misfit": misfit,
     }
+    # the whole timed step at chip level: the algorithmic bytes of the node-sweeps that the
+    # sweeps performed in a step (the per-launch step above performs the same ones: a
+    # chunk's launches do not depend on the streams) over the timed step time
+    swept_step = sum(k["bytes_swept"] for k in kern.values())
+    out["roofline_step"] = {"bytes_swept_per_step": swept_step,
+                            "achieved_GBps": round(swept_step / (ms_step * 1e-3) / 1e9, 1),
+                            "frac": round(swept_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "streams": args.streams}
     if plain:
         bytes_solve = 8.0 * N * (If * (8.0 + 4.0 / B) + Ia * 12.0)
         out["plain_fp64"] = {"value": round(plain_rate, 3), "unit": "source-solves/s",
