@@ -1466,8 +1466,8 @@ namespace pfast {
 // kernel), all of the column-window schedule.  Variant 0 is the default; the others
 // stay selectable for measurement and for the autotuner.  The adjoint keeps a
 // (lambda, T) pair per queued value, so its register count grows fast with the batch
-// depth (8x8 tiles: 46 VGPRs at L2, 64 at L4, 83 at L6, which leaves 6 waves per SIMD
-// instead of 8).  Tiles of 1024 threads (8x16, 16x8) and batch depth 8 measured 1.4-2.8x slower
+// depth (8x8 tiles: 52 VGPRs at L2, 71 at L4, 90 at L6, which leaves 7 or 5 waves per
+// SIMD instead of 8).  Tiles of 1024 threads (8x16, 16x8) and batch depth 8 measured 1.4-2.8x slower
 // on the 400-source workload and are not built.
 #define PFAST_VARIANTS(X)                                                                 \
     X(8, 8, 4, 2) X(8, 8, 4, 4) X(8, 8, 6, 6) X(8, 8, 2, 2) X(8, 8, 6, 2) X(8, 4, 4, 4)     \
