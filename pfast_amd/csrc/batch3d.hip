@@ -597,11 +597,11 @@ __device__ __forceinline__ void fwd_tile(const SweepArgs &a, const TileCtx &t, i
         flow_wait(*fw, L - 1);
         asm volatile("" ::: "memory");
     }
-    // the sweep starts at step pb0 <= k0 (a multiple of two batches, for the queue and
-    // parity alternation), where the window holds levels m0-1 (as this sweep leaves it:
+    // the sweep starts at step pb0 <= k0 (a multiple of the batch depth, where every
+    // batch of the column starts), where the window holds levels m0-1 (as this sweep leaves it:
     // below k0 that is the stored value), m0 and m0+1 of level m0 = pb0 - pipj
     // (the dataflow kernels start at step 0: level m0-1 is outside the column)
-    const int pb0 = FLOW ? 0 : k0 / (2 * L) * (2 * L);
+    const int pb0 = FLOW ? 0 : k0 / L * L;
     const int m0 = pb0 - l.pipj;
     auto lev = [&](int m) { return l.valid && (unsigned)m < (unsigned)nz; };
     float vprev, vcur, vnext;
@@ -820,7 +820,7 @@ __device__ __forceinline__ void adj_tile(const SweepArgs &a, const TileCtx &t, i
         asm volatile("" ::: "memory");
     }
     // the window at step pb0 (as in fwd_tile)
-    const int pb0 = FLOW ? 0 : k0 / (2 * L) * (2 * L);
+    const int pb0 = FLOW ? 0 : k0 / L * L;
     const int m0 = pb0 - l.pipj;
     auto lev = [&](int m) { return l.valid && (unsigned)m < (unsigned)nz; };
     const float2 zero = make_float2(0.0f, 0.0f);
